@@ -1,0 +1,309 @@
+// ppa_api.cpp -- the C ABI (include/ppa.h) over the gfx950 kernels.
+//
+// Replaces the cuPPA class (src/PA/cuPPA/cuPPA.cuh:84-105): the constructor's
+// (N, M, p) become a ppa_graph_desc, the per-call cudaMalloc/cudaFree of the
+// graph (cuPPA_1.cu:54-60,247-250) become caller-owned buffers, the stage
+// loop (cuPPA_1.cu:112-226) becomes one persistent launch, and the
+// printf-and-continue error handling (cuPPA_1.cu:42-46,106-110) becomes
+// status codes.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+
+#include "../../include/ppa.h"
+#include "ppa_kernels.hpp"
+
+struct ppa_ctx {
+    int device = 0;
+    int num_cus = 0;
+    int wall_clock_khz = 0;
+    ppa_tuning tuning{};
+    ppa::DeviceScratch *scratch = nullptr;  // device
+    uint32_t *degree_scratch = nullptr;     // device, grown on demand
+    uint64_t degree_scratch_len = 0;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    ppa_stats last{};
+    bool pending = false;  // an async generate awaits ppa_ctx_sync
+    uint64_t pending_n = 0;
+    uint32_t pending_d = 0;
+};
+
+namespace {
+
+constexpr uint32_t kDefaultBlock = 256;
+constexpr uint32_t kDefaultChunk = 256;
+constexpr uint32_t kDefaultWatchdogMs = 60000;
+
+class DeviceGuard {
+public:
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev_) != hipSuccess) prev_ = -1;
+        ok_ = hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceGuard()
+    {
+        if (prev_ >= 0) (void)hipSetDevice(prev_);
+    }
+    bool ok() const { return ok_; }
+
+private:
+    int prev_ = -1;
+    bool ok_ = false;
+};
+
+#define PPA_HIP(call)                               \
+    do {                                            \
+        if ((call) != hipSuccess) return PPA_ERR_DEVICE; \
+    } while (0)
+
+int resolve_tuning(const ppa_ctx *ctx, uint32_t d, uint64_t nodes, uint32_t *blocks, uint32_t *threads,
+                   uint32_t *chunk)
+{
+    const ppa_tuning &tu = ctx->tuning;
+    uint32_t th = tu.block_threads ? tu.block_threads : kDefaultBlock;
+    if (th % 64 != 0 || th > 256) return PPA_ERR_INVALID;  // kernels are built for <= 256
+    uint32_t ch = tu.chunk_nodes ? tu.chunk_nodes : kDefaultChunk;
+    if (ch % 64 != 0) return PPA_ERR_INVALID;
+    int per_cu = 0;
+    PPA_HIP(ppa::generate_occupancy(d, th, &per_cu));
+    if (per_cu < 1) per_cu = 1;
+    if (tu.blocks_per_cu && (int)tu.blocks_per_cu < per_cu) per_cu = (int)tu.blocks_per_cu;
+    uint64_t g = (uint64_t)per_cu * (uint64_t)ctx->num_cus;
+    // no more wavefronts than there are chunks of work
+    const uint64_t waves_needed = (nodes + ch - 1) / ch;
+    const uint64_t blocks_needed = (waves_needed + th / 64 - 1) / (th / 64);
+    if (g > blocks_needed) g = blocks_needed;
+    if (tu.max_blocks && g > tu.max_blocks) g = tu.max_blocks;
+    if (g < 1) g = 1;
+    *blocks = (uint32_t)g;
+    *threads = th;
+    *chunk = ch;
+    return PPA_OK;
+}
+
+int finish(ppa_ctx *ctx, hipStream_t stream)
+{
+    PPA_HIP(hipStreamSynchronize(stream));
+    ppa::DeviceScratch host{};
+    PPA_HIP(hipMemcpy(&host, ctx->scratch, sizeof(host), hipMemcpyDeviceToHost));
+    ppa_stats &s = ctx->last;
+    s.nodes = host.stats[ppa::kStatNodes];
+    s.slots = host.stats[ppa::kStatSlots];
+    s.draws = host.stats[ppa::kStatDraws];
+    s.retries = host.stats[ppa::kStatRetries];
+    s.prefetch_misses = host.stats[ppa::kStatPrefetchMisses];
+    s.polls = host.stats[ppa::kStatPolls];
+    s.stalls = host.stats[ppa::kStatStalls];
+    s.claims = host.stats[ppa::kStatClaims];
+    s.sleeps = host.stats[ppa::kStatSleeps];
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) s.kernel_ms = ms;
+    if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]) == hipSuccess) s.total_ms = ms;
+    ctx->pending = false;
+    if (host.status != 0) return host.status;
+    // every generated node must have been finished by exactly one lane
+    const uint64_t expect = ctx->pending_n - (uint64_t)ctx->pending_d - 1;
+    if (s.nodes != expect) return PPA_ERR_INTERNAL;
+    return PPA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ppa_strerror(int code)
+{
+    switch (code) {
+    case PPA_OK: return "ok";
+    case PPA_ERR_INVALID: return "invalid argument";
+    case PPA_ERR_RETRY: return "retry limit exceeded for a slot";
+    case PPA_ERR_NOMEM: return "out of memory";
+    case PPA_ERR_DEVICE: return "HIP runtime error";
+    case PPA_ERR_TIMEOUT: return "kernel watchdog expired";
+    case PPA_ERR_BUDGET: return "request exceeds the memory budget";
+    case PPA_ERR_NOT_FOUND: return "unknown request";
+    case PPA_ERR_CANCELLED: return "request cancelled";
+    case PPA_ERR_AGAIN: return "not ready, try again";
+    case PPA_ERR_INTERNAL: return "internal error";
+    default: return "unknown error";
+    }
+}
+
+uint64_t ppa_p_threshold(double p)
+{
+    if (!(p > 0.0)) return 0;
+    if (p >= 1.0) return 1ull << 32;
+    return (uint64_t)(p * 4294967296.0);  // exact scaling, then floor
+}
+
+int ppa_check_desc(const ppa_graph_desc *desc)
+{
+    if (!desc) return PPA_ERR_INVALID;
+    if (desc->d < 1 || desc->d > PPA_MAX_D) return PPA_ERR_INVALID;
+    if (desc->n < (uint64_t)desc->d + 1 || desc->n > PPA_MAX_N) return PPA_ERR_INVALID;
+    if (desc->p_threshold > (1ull << 32)) return PPA_ERR_INVALID;
+    return PPA_OK;
+}
+
+int ppa_ctx_create(ppa_ctx **out, int device)
+{
+    if (!out) return PPA_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return PPA_ERR_DEVICE;
+    if (device < 0 || device >= count) return PPA_ERR_INVALID;
+    DeviceGuard g(device);
+    if (!g.ok()) return PPA_ERR_DEVICE;
+    ppa_ctx *c = new (std::nothrow) ppa_ctx;
+    if (!c) return PPA_ERR_NOMEM;
+    c->device = device;
+    if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        hipDeviceGetAttribute(&c->wall_clock_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess) {
+        delete c;
+        return PPA_ERR_DEVICE;
+    }
+    if (c->wall_clock_khz <= 0) c->wall_clock_khz = 100000;
+    if (hipMalloc(&c->scratch, sizeof(ppa::DeviceScratch)) != hipSuccess) {
+        delete c;
+        return PPA_ERR_NOMEM;
+    }
+    for (auto &e : c->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            ppa_ctx_destroy(c);
+            return PPA_ERR_DEVICE;
+        }
+    }
+    *out = c;
+    return PPA_OK;
+}
+
+int ppa_ctx_destroy(ppa_ctx *ctx)
+{
+    if (!ctx) return PPA_ERR_INVALID;
+    DeviceGuard g(ctx->device);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->degree_scratch) (void)hipFree(ctx->degree_scratch);
+    delete ctx;
+    return PPA_OK;
+}
+
+int ppa_ctx_set_tuning(ppa_ctx *ctx, const ppa_tuning *t)
+{
+    if (!ctx || !t) return PPA_ERR_INVALID;
+    if (t->block_threads && (t->block_threads % 64 != 0 || t->block_threads > 256)) return PPA_ERR_INVALID;
+    if (t->chunk_nodes && t->chunk_nodes % 64 != 0) return PPA_ERR_INVALID;
+    ctx->tuning = *t;
+    return PPA_OK;
+}
+
+int ppa_ctx_get_tuning(ppa_ctx *ctx, ppa_tuning *out)
+{
+    if (!ctx || !out) return PPA_ERR_INVALID;
+    *out = ctx->tuning;
+    return PPA_OK;
+}
+
+int ppa_get_stats(ppa_ctx *ctx, ppa_stats *out)
+{
+    if (!ctx || !out) return PPA_ERR_INVALID;
+    *out = ctx->last;
+    return PPA_OK;
+}
+
+int ppa_generate_async(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets, int64_t *d_row_ptr,
+                       void *stream_)
+{
+    if (!ctx || !d_targets) return PPA_ERR_INVALID;
+    int rc = ppa_check_desc(desc);
+    if (rc) return rc;
+    if (((uintptr_t)d_targets & 15u) != 0) return PPA_ERR_INVALID;
+    DeviceGuard g(ctx->device);
+    if (!g.ok()) return PPA_ERR_DEVICE;
+    hipStream_t stream = (hipStream_t)stream_;
+    const uint64_t n = desc->n;
+    const uint32_t d = desc->d;
+    uint32_t blocks = 0, threads = 0, chunk = 0;
+    rc = resolve_tuning(ctx, d, n - d - 1, &blocks, &threads, &chunk);
+    if (rc) return rc;
+
+    ctx->last = ppa_stats{};
+    ctx->last.grid_blocks = blocks;
+    ctx->last.block_threads = threads;
+    PPA_HIP(hipEventRecord(ctx->ev[0], stream));
+    PPA_HIP(ppa::launch_reset_scratch(ctx->scratch, (uint64_t)d + 1, stream));
+    ppa::FillArgs fa{d_targets, d_row_ptr, n, d, 0};
+    PPA_HIP(ppa::launch_fill(fa, stream));
+    PPA_HIP(hipEventRecord(ctx->ev[1], stream));
+    if (n > (uint64_t)d + 1) {
+        ppa::GenArgs ga{};
+        ga.G = d_targets;
+        ga.n = n;
+        ga.T = desc->p_threshold;
+        ga.key0 = (uint32_t)desc->seed;
+        ga.key1 = (uint32_t)(desc->seed >> 32);
+        ga.chunk = chunk;
+        const uint32_t wd = ctx->tuning.watchdog_ms ? ctx->tuning.watchdog_ms : kDefaultWatchdogMs;
+        ga.watchdog_ticks = (uint64_t)wd * (uint64_t)ctx->wall_clock_khz;
+        ga.scratch = ctx->scratch;
+        PPA_HIP(ppa::launch_generate(d, ga, blocks, threads, stream));
+    }
+    PPA_HIP(hipEventRecord(ctx->ev[2], stream));
+    ctx->pending = true;
+    ctx->pending_n = n;
+    ctx->pending_d = d;
+    return PPA_OK;
+}
+
+int ppa_ctx_sync(ppa_ctx *ctx, void *stream)
+{
+    if (!ctx) return PPA_ERR_INVALID;
+    if (!ctx->pending) return PPA_OK;
+    DeviceGuard g(ctx->device);
+    if (!g.ok()) return PPA_ERR_DEVICE;
+    return finish(ctx, (hipStream_t)stream);
+}
+
+int ppa_generate(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets, int64_t *d_row_ptr,
+                 void *stream)
+{
+    int rc = ppa_generate_async(ctx, desc, d_targets, d_row_ptr, stream);
+    if (rc) return rc;
+    return ppa_ctx_sync(ctx, stream);
+}
+
+int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_targets,
+                         uint32_t *d_degree, uint64_t *d_hist, uint64_t hist_len, uint64_t *d_max_degree,
+                         void *stream_)
+{
+    if (!ctx || !d_targets) return PPA_ERR_INVALID;
+    int rc = ppa_check_desc(desc);
+    if (rc) return rc;
+    if (d_hist == nullptr && hist_len != 0) return PPA_ERR_INVALID;
+    DeviceGuard g(ctx->device);
+    if (!g.ok()) return PPA_ERR_DEVICE;
+    hipStream_t stream = (hipStream_t)stream_;
+    uint32_t *deg = d_degree;
+    if (!deg) {
+        if (ctx->degree_scratch_len < desc->n) {
+            if (ctx->degree_scratch) PPA_HIP(hipFree(ctx->degree_scratch));
+            ctx->degree_scratch = nullptr;
+            ctx->degree_scratch_len = 0;
+            if (hipMalloc(&ctx->degree_scratch, desc->n * sizeof(uint32_t)) != hipSuccess) return PPA_ERR_NOMEM;
+            ctx->degree_scratch_len = desc->n;
+        }
+        deg = ctx->degree_scratch;
+    }
+    PPA_HIP(ppa::launch_reset_scratch(ctx->scratch, 0, stream));
+    PPA_HIP(ppa::launch_degree(d_targets, desc->n, desc->d, deg, d_hist, hist_len, ctx->scratch, stream));
+    if (d_max_degree)
+        PPA_HIP(hipMemcpyAsync(d_max_degree, &ctx->scratch->max_degree, sizeof(uint64_t),
+                               hipMemcpyDeviceToDevice, stream));
+    PPA_HIP(hipStreamSynchronize(stream));
+    return PPA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// ppa_model.hpp -- keyed draws of the copy model, shared by host and device.
+//
+// The reference draws with a per-thread cuRAND XORWOW stream seeded from the
+// wall clock (cuRandom.cu:47-52, cuPPA_1.cu:39-40): `curand % u` for the node,
+// `curand_uniform` against p for the coin, `curand % M` for the copied slot
+// (cuPPA_RRP_1.cu:70-85). Its serial path draws one 64-bit value over the
+// M*t (node, slot) pairs and a double coin (SerialCopyModel3.cpp:62-67).
+// Here every attempt is one stateless Philox-4x32-10 call keyed by
+// (seed; t, k, r), written from Salmon et al., "Parallel random numbers: as
+// easy as 1, 2, 3" (SC'11): x0 is the coin, (x1, x2) the 64-bit (node, slot)
+// draw, which keeps the serial path's single draw over the t*d pairs.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define PPA_HD __host__ __device__ __forceinline__
+#else
+#define PPA_HD inline
+#endif
+
+namespace ppa {
+
+constexpr int32_t kEmpty = -1;  // an unfilled slot (SerialCopyModel.cpp:41 uses -1 too)
+
+struct Draw {
+    uint32_t x0, x1, x2, x3;
+};
+
+PPA_HD uint32_t mul_hi32(uint32_t a, uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(a, b);
+#else
+    return (uint32_t)(((uint64_t)a * b) >> 32);
+#endif
+}
+
+// Philox-4x32-10: ten rounds of the 4x32 S-box with key schedule
+// (k0, k1) += (0x9E3779B9, 0xBB67AE85) between rounds.
+PPA_HD Draw philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        const uint32_t lo_a = 0xD2511F53u * c0;
+        const uint32_t hi_a = mul_hi32(0xD2511F53u, c0);
+        const uint32_t lo_b = 0xCD9E8D57u * c2;
+        const uint32_t hi_b = mul_hi32(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi_b ^ c1 ^ k0;
+        const uint32_t n2 = hi_a ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo_b;
+        c2 = n2;
+        c3 = lo_a;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return Draw{c0, c1, c2, c3};
+}
+
+PPA_HD uint64_t mul_hi64(uint64_t a, uint64_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+// Attempt r of slot k of node t: the coin and the (node, slot) pair.
+struct Attempt {
+    bool direct;
+    uint64_t pair;  // R = u*d + j, uniform over [0, t*d)
+};
+
+PPA_HD Attempt attempt(uint32_t key0, uint32_t key1, uint64_t T, uint64_t t, uint32_t k, uint32_t r,
+                       uint64_t td)
+{
+    const Draw x = philox((uint32_t)t, (uint32_t)(t >> 32), k, r, key0, key1);
+    Attempt a;
+    a.direct = (uint64_t)x.x0 < T;
+    a.pair = mul_hi64(((uint64_t)x.x1 << 32) | x.x2, td);
+    return a;
+}
+
+// Seed clique on nodes 0..d: node i's slot j names j + (j >= i).
+PPA_HD int32_t clique_target(uint64_t i, uint32_t j) { return (int32_t)(j + (j >= i ? 1u : 0u)); }
+
+}  // namespace ppa

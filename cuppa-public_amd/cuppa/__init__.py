@@ -1,0 +1,256 @@
+"""Python binding of the MI355X copy-model generator (ctypes over include/ppa.h).
+
+The C ABI in ``lib/libppa.so`` is the product; this module is a thin host-side
+wrapper used by the tests, the benchmark and Python callers. Device memory and
+streams come from PyTorch (plumbing only). Nothing here computes graph values:
+if the native library is missing, every entry point raises.
+
+It mirrors the reference's generator object (src/PA/cuPPA/cuPPA.cuh:84-105):
+``CuPPA(N, M, p)`` with ``generate_graph()`` / ``generate_graph_adaptive()``
+(cuPPA_1.cu:9-254, cuPPA_2.cu:9-241) and ``degree_distribution()``
+(cuPPA.cu:103-165), except that the graph is returned instead of freed.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libppa.so")
+
+PPA_OK = 0
+PPA_MAX_D = 16
+PPA_MAX_N = 0x7FFFFFFF
+PPA_MAX_RETRY = 1 << 24
+
+ERRORS = {
+    -1: "PPA_ERR_INVALID", -2: "PPA_ERR_RETRY", -3: "PPA_ERR_NOMEM", -4: "PPA_ERR_DEVICE",
+    -5: "PPA_ERR_TIMEOUT", -6: "PPA_ERR_BUDGET", -7: "PPA_ERR_NOT_FOUND", -8: "PPA_ERR_CANCELLED",
+    -9: "PPA_ERR_AGAIN", -10: "PPA_ERR_INTERNAL",
+}
+
+
+class PPAError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        self.code = code
+        msg = f"{ERRORS.get(code, code)}: {strerror(code)}"
+        super().__init__(f"{what}: {msg}" if what else msg)
+
+
+class GraphDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("d", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("p_threshold", ctypes.c_uint64), ("seed", ctypes.c_uint64),
+                ("targets_offset", ctypes.c_uint64), ("row_ptr_offset", ctypes.c_uint64)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("block_threads", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
+                ("chunk_nodes", ctypes.c_uint32), ("max_blocks", ctypes.c_uint32),
+                ("watchdog_ms", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_uint64), ("slots", ctypes.c_uint64), ("draws", ctypes.c_uint64),
+                ("retries", ctypes.c_uint64), ("prefetch_misses", ctypes.c_uint64),
+                ("polls", ctypes.c_uint64), ("stalls", ctypes.c_uint64), ("claims", ctypes.c_uint64),
+                ("sleeps", ctypes.c_uint64), ("grid_blocks", ctypes.c_uint64),
+                ("block_threads", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+
+# (name, restype, argtypes) of every exported entry point
+_VP = ctypes.c_void_p
+_SIGNATURES = [
+    ("ppa_strerror", ctypes.c_char_p, [ctypes.c_int]),
+    ("ppa_p_threshold", ctypes.c_uint64, [ctypes.c_double]),
+    ("ppa_check_desc", ctypes.c_int, [ctypes.POINTER(GraphDesc)]),
+    ("ppa_ctx_create", ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int]),
+    ("ppa_ctx_destroy", ctypes.c_int, [_VP]),
+    ("ppa_ctx_set_tuning", ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
+    ("ppa_ctx_get_tuning", ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
+    ("ppa_get_stats", ctypes.c_int, [_VP, ctypes.POINTER(Stats)]),
+    ("ppa_generate", ctypes.c_int, [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP]),
+    ("ppa_generate_async", ctypes.c_int, [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP]),
+    ("ppa_ctx_sync", ctypes.c_int, [_VP, _VP]),
+    ("ppa_degree_histogram", ctypes.c_int,
+     [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP, ctypes.c_uint64, _VP, _VP]),
+]
+
+
+def lib():
+    """Loads lib/libppa.so; raises if it was not built (no fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"native library missing: {LIB_PATH} (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, res, args in _SIGNATURES:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    return [name for name, _, _ in _SIGNATURES]
+
+
+def strerror(code: int) -> str:
+    try:
+        return lib().ppa_strerror(code).decode()
+    except ImportError:
+        return ERRORS.get(code, str(code))
+
+
+def p_threshold(p: float) -> int:
+    return int(lib().ppa_p_threshold(p))
+
+
+def make_desc(n: int, d: int, p: float | None = None, seed: int = 0, p_thresh: int | None = None,
+              targets_offset: int = 0, row_ptr_offset: int = 0) -> GraphDesc:
+    if p_thresh is None:
+        if p is None:
+            raise ValueError("give p or p_thresh")
+        p_thresh = p_threshold(p)
+    return GraphDesc(n, d, 0, p_thresh, seed & 0xFFFFFFFFFFFFFFFF, targets_offset, row_ptr_offset)
+
+
+def _check(rc: int, what: str):
+    if rc != PPA_OK:
+        raise PPAError(rc, what)
+
+
+def _stream_handle(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+@dataclass
+class Graph:
+    n: int
+    d: int
+    targets: "object"  # torch.int32 [n*d] on the device
+    row_ptr: "object"  # torch.int64 [n+1] on the device, or None
+
+
+class Context:
+    """One device context (ppa_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        self.device = device
+        _check(lib().ppa_ctx_create(ctypes.byref(self._h), device), "ppa_ctx_create")
+
+    def close(self):
+        if self._h:
+            lib().ppa_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_tuning(self, block_threads=0, blocks_per_cu=0, chunk_nodes=0, max_blocks=0, watchdog_ms=0):
+        t = Tuning(block_threads, blocks_per_cu, chunk_nodes, max_blocks, watchdog_ms)
+        _check(lib().ppa_ctx_set_tuning(self._h, ctypes.byref(t)), "ppa_ctx_set_tuning")
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().ppa_get_stats(self._h, ctypes.byref(s)), "ppa_get_stats")
+        return s.as_dict()
+
+    def generate_into(self, desc: GraphDesc, targets, row_ptr=None, stream=None, sync=True):
+        """targets / row_ptr: device tensors (int32 [n*d], int64 [n+1]) or raw pointers."""
+        tp = targets.data_ptr() if hasattr(targets, "data_ptr") else int(targets)
+        rp = (row_ptr.data_ptr() if hasattr(row_ptr, "data_ptr") else int(row_ptr)) if row_ptr is not None else None
+        fn = lib().ppa_generate if sync else lib().ppa_generate_async
+        _check(fn(self._h, ctypes.byref(desc), ctypes.c_void_p(tp), ctypes.c_void_p(rp) if rp else None,
+                  _stream_handle(stream)), "ppa_generate")
+
+    def sync(self, stream=None):
+        _check(lib().ppa_ctx_sync(self._h, _stream_handle(stream)), "ppa_ctx_sync")
+
+    def generate(self, n: int, d: int, p: float | None = None, seed: int = 0, p_thresh: int | None = None,
+                 row_ptr: bool = True, stream=None) -> Graph:
+        import torch
+        desc = make_desc(n, d, p, seed, p_thresh)
+        _check(lib().ppa_check_desc(ctypes.byref(desc)), "ppa_check_desc")
+        dev = torch.device("cuda", self.device)
+        targets = torch.empty(n * d, dtype=torch.int32, device=dev)
+        rp = torch.empty(n + 1, dtype=torch.int64, device=dev) if row_ptr else None
+        self.generate_into(desc, targets, rp, stream)
+        return Graph(n, d, targets, rp)
+
+    def degree_histogram(self, g: Graph, hist_len: int | None = None, want_degree: bool = False, stream=None):
+        """Returns (hist uint64 tensor, max_degree int, degree tensor or None)."""
+        import torch
+        dev = torch.device("cuda", self.device)
+        desc = make_desc(g.n, g.d, p_thresh=0)
+        deg = torch.empty(g.n, dtype=torch.int32, device=dev) if want_degree else None
+        mx = torch.zeros(1, dtype=torch.int64, device=dev)
+        if hist_len is None:
+            # first pass only for the maximum degree
+            _check(lib().ppa_degree_histogram(self._h, ctypes.byref(desc), ctypes.c_void_p(g.targets.data_ptr()),
+                                              ctypes.c_void_p(deg.data_ptr()) if deg is not None else None,
+                                              None, 0, ctypes.c_void_p(mx.data_ptr()), _stream_handle(stream)),
+                   "ppa_degree_histogram")
+            hist_len = int(mx.item()) + 1
+        hist = torch.zeros(hist_len, dtype=torch.int64, device=dev)
+        _check(lib().ppa_degree_histogram(self._h, ctypes.byref(desc), ctypes.c_void_p(g.targets.data_ptr()),
+                                          ctypes.c_void_p(deg.data_ptr()) if deg is not None else None,
+                                          ctypes.c_void_p(hist.data_ptr()), hist_len,
+                                          ctypes.c_void_p(mx.data_ptr()), _stream_handle(stream)),
+               "ppa_degree_histogram")
+        return hist, int(mx.item()), deg
+
+
+class CuPPA:
+    """Mirror of the reference's ``cuPPA`` object (cuPPA.cuh:84-105).
+
+    ``CuPPA(N, M, p)`` keeps the constructor's argument meaning: N nodes, M
+    out-edges per new node, p the probability of a direct edge (the ``-a``
+    flag, src/main.cu:198). ``seed`` keys the Philox draws (the reference
+    seeds with ``time(0)``, cuPPA_1.cu:39).
+    """
+
+    def __init__(self, N: int, M: int, p: float, seed: int = 0, device: int = 0):
+        self._N, self._M, self._p, self.seed = N, M, p, seed
+        self.ctx = Context(device)
+        self.graph: Graph | None = None
+
+    def generate_graph(self) -> Graph:
+        self.graph = self.ctx.generate(self._N, self._M, self._p, self.seed)
+        return self.graph
+
+    # The adaptive stage sizing of cuPPA_2.cu:181-198 has no counterpart to
+    # tune: one persistent launch covers every stage; the output is the same.
+    generate_graph_adaptive = generate_graph
+
+    def degree_distribution(self) -> dict:
+        if self.graph is None:
+            raise RuntimeError("generate_graph() first")
+        hist, _, _ = self.ctx.degree_histogram(self.graph)
+        h = hist.cpu().tolist()
+        return {deg: c for deg, c in enumerate(h) if c > 0}

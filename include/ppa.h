@@ -1,0 +1,143 @@
+/*
+ * ppa.h -- C ABI of the MI355X copy-model preferential-attachment generator.
+ *
+ * This is the drop-in boundary for cuPPA's GPU generator. The reference has no
+ * FFI; its path sits behind the C++ class cuPPA (src/PA/cuPPA/cuPPA.cuh:84-105)
+ * whose generateGraph / generateGraphAdaptive (cuPPA_1.cu:9-254,
+ * cuPPA_2.cu:9-241) allocate the graph, run the stage loop, optionally write
+ * the degree distribution (cuPPA.cu:103-165) and free everything. The entry
+ * points below replace that class one for one, with these differences:
+ *   - the caller owns the output buffers (device pointers) and gets the graph
+ *     back instead of having it freed inside the call;
+ *   - no exceptions or exit() cross the ABI: every function returns an int
+ *     status (PPA_OK = 0, negative = error, see ppa_strerror);
+ *   - the device is never reset (cuPPA_1.cu:11 calls cudaDeviceReset);
+ *   - randomness is keyed (seed, node, slot, retry) Philox-4x32-10, so the
+ *     output does not depend on launch shape or scheduling.
+ *
+ * Model (the copy model of src/PA/SPA/SerialCopyModel3.cpp:38-78, keyed):
+ *   nodes 0..d form a clique (node i's slot j = j + (j >= i)); every node
+ *   t = d+1 .. n-1 fills slots k = 0..d-1 in order; attempt r of slot k draws
+ *   x = Philox4x32_10(key = (lo32 seed, hi32 seed), ctr = (lo32 t, hi32 t, k, r)),
+ *   direct iff x0 < p_threshold, R = mulhi64((x1 << 32) | x2, t*d),
+ *   u = R / d, j = R % d, candidate = direct ? u : target[u*d + j],
+ *   and the attempt is rejected (r + 1) iff the candidate already fills one of
+ *   slots 0..k-1 of node t.
+ *
+ * Threading: a context is bound to one device. Calls on one context must not
+ * overlap; different contexts may be used from different host threads.
+ * Streams are passed as `void*` holding a hipStream_t (NULL = default stream).
+ */
+#ifndef PPA_H_
+#define PPA_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+    PPA_OK = 0,
+    PPA_ERR_INVALID = -1,   /* bad argument (sizes, d, pointers) */
+    PPA_ERR_RETRY = -2,     /* a slot exceeded PPA_MAX_RETRY rejected attempts */
+    PPA_ERR_NOMEM = -3,     /* device or host allocation failed */
+    PPA_ERR_DEVICE = -4,    /* HIP runtime error */
+    PPA_ERR_TIMEOUT = -5,   /* the kernel watchdog expired; the output is incomplete */
+    PPA_ERR_BUDGET = -6,    /* request exceeds the configured memory budget */
+    PPA_ERR_NOT_FOUND = -7, /* unknown request id */
+    PPA_ERR_CANCELLED = -8, /* request was cancelled */
+    PPA_ERR_AGAIN = -9,     /* not ready yet / queue full, try again */
+    PPA_ERR_INTERNAL = -10
+};
+
+#define PPA_MAX_RETRY (1u << 24)
+#define PPA_MAX_D 16u            /* largest out-degree the generator supports */
+#define PPA_MAX_N 0x7fffffffull  /* targets are int32 (DataTypes.h:11-18) */
+
+/* ---- graph descriptor ---------------------------------------------------- */
+/* One copy-model graph. n >= d + 1, 1 <= d <= PPA_MAX_D, p_threshold <= 2^32.
+ * targets_offset / row_ptr_offset are element offsets into the concatenated
+ * output buffers; only ppa_generate_batch reads them. */
+typedef struct ppa_graph_desc {
+    uint64_t n;
+    uint32_t d;
+    uint32_t reserved;
+    uint64_t p_threshold;   /* min(2^32, floor(p * 2^32)); see ppa_p_threshold */
+    uint64_t seed;          /* Philox key */
+    uint64_t targets_offset;
+    uint64_t row_ptr_offset;
+} ppa_graph_desc;
+
+/* Launch shape of the generator. Any shape gives the same output. */
+typedef struct ppa_tuning {
+    uint32_t block_threads;  /* workgroup size, multiple of 64, <= 1024 (0 = default) */
+    uint32_t blocks_per_cu;  /* resident workgroups per CU (0 = occupancy maximum) */
+    uint32_t chunk_nodes;    /* nodes a wavefront claims at once, multiple of 64 (0 = default) */
+    uint32_t max_blocks;     /* cap on the persistent grid (0 = no cap) */
+    uint32_t watchdog_ms;    /* kernel deadline (0 = default 60000) */
+    uint32_t reserved[3];
+} ppa_tuning;
+
+/* Counters of the last generate call (cuPPA_1.cu:84-87 timing CSV and the
+ * max-queue statistic of cuPPA_RRP_1.cu:191-208 become these). */
+typedef struct ppa_stats {
+    uint64_t nodes;          /* nodes generated (t >= d+1) */
+    uint64_t slots;          /* slots filled */
+    uint64_t draws;          /* Philox calls */
+    uint64_t retries;        /* rejected attempts (duplicate targets) */
+    uint64_t prefetch_misses;/* copy reads that found the source slot still empty */
+    uint64_t polls;          /* coherent re-reads of an empty source slot */
+    uint64_t stalls;         /* polls that found the slot still empty */
+    uint64_t claims;         /* work-claim atomics */
+    uint64_t sleeps;         /* wavefront back-off sleeps */
+    uint64_t grid_blocks;    /* persistent workgroups launched */
+    uint64_t block_threads;
+    double kernel_ms;        /* device time of the generation kernel */
+    double total_ms;         /* device time of the whole call (fill + generate) */
+} ppa_stats;
+
+typedef struct ppa_ctx ppa_ctx;
+
+/* ---- utilities ------------------------------------------------------------ */
+const char *ppa_strerror(int code);
+/* min(2^32, floor(p * 2^32)); p <= 0 -> 0. */
+uint64_t ppa_p_threshold(double p);
+/* Validates a descriptor (same checks as ppa_generate). */
+int ppa_check_desc(const ppa_graph_desc *desc);
+
+/* ---- context (replaces the cuPPA object, cuPPA.cuh:84-105) ---------------- */
+int ppa_ctx_create(ppa_ctx **out, int device);
+int ppa_ctx_destroy(ppa_ctx *ctx);
+int ppa_ctx_set_tuning(ppa_ctx *ctx, const ppa_tuning *tuning);
+int ppa_ctx_get_tuning(ppa_ctx *ctx, ppa_tuning *out);
+int ppa_get_stats(ppa_ctx *ctx, ppa_stats *out);
+
+/* ---- single graph: replaces cuPPA::generateGraph (cuPPA_1.cu:9-254) and
+ *      cuPPA::generateGraphAdaptive (cuPPA_2.cu:9-241) ---------------------- */
+/* Writes d_targets[0 .. n*d) (int32, row-major [n][d]) and, if d_row_ptr is
+ * not NULL, d_row_ptr[0 .. n] = t*d. Enqueues on `stream`, waits for it, and
+ * returns the kernel's status. */
+int ppa_generate(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets,
+                 int64_t *d_row_ptr, void *stream);
+/* Same work without the final wait; the status is read by ppa_ctx_sync. */
+int ppa_generate_async(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets,
+                       int64_t *d_row_ptr, void *stream);
+int ppa_ctx_sync(ppa_ctx *ctx, void *stream);
+
+/* ---- degree pass: replaces cuPPA::printDegreeDistribution (cuPPA.cu:103-165)
+ *      and PAHelper::outputSCMDegreeDistribution (PAHelper.hpp:15-67) ------- */
+/* deg(v) = d + #{slots of nodes t >= d+1 naming v}. d_degree (nullable,
+ * uint32[n]) receives deg; d_hist (nullable, uint64[hist_len]) receives
+ * hist[x] = #{v : deg(v) == x} for x < hist_len; d_max_degree (nullable,
+ * one uint64) receives max deg. Waits for the stream. */
+int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_targets,
+                         uint32_t *d_degree, uint64_t *d_hist, uint64_t hist_len,
+                         uint64_t *d_max_degree, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PPA_H_ */

@@ -1,0 +1,78 @@
+"""GPU parity: the HIP generator against the CPU oracle, bit for bit."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(ctx, n, d, T, seed):
+    import cuppa
+    g = ctx.generate(n, d, p_thresh=T, seed=seed)
+    return g.targets.cpu().numpy(), g.row_ptr.cpu().numpy()
+
+
+CASES = [
+    # n, d, p, seed
+    (2, 1, 0.5, 1),          # clique only
+    (3, 1, 0.5, 1),
+    (6, 4, 0.5, 3),          # one generated node
+    (1000, 1, 0.0, 5),
+    (1000, 1, 1.0, 5),
+    (5000, 3, 0.0, 9),       # copies only: every target in the seed clique
+    (20000, 8, 0.5, 7),
+    (20000, 16, 0.2, 11),
+    (50000, 2, 0.9, 123),
+    (100000, 5, 0.35, 2**40 + 17),
+]
+
+
+@pytest.mark.parametrize("n,d,p,seed", CASES)
+def test_small_graphs_bit_exact(ctx, oracle_mod, n, d, p, seed):
+    T = oracle_mod.p_threshold(p)
+    want, _, _ = oracle_mod.generate(n, d, T, seed)
+    got, rp = _gpu(ctx, n, d, T, seed)
+    assert np.array_equal(got, want)
+    assert np.array_equal(rp, np.arange(n + 1, dtype=np.int64) * d)
+    st = ctx.stats()
+    assert st["nodes"] == n - d - 1
+
+
+def test_cfg1_bit_exact(ctx, oracle_mod):
+    n, d, p, seed = 1_000_000, 4, 0.5, 42
+    T = oracle_mod.p_threshold(p)
+    want, _, _ = oracle_mod.generate(n, d, T, seed)
+    got, _ = _gpu(ctx, n, d, T, seed)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(block_threads=64, chunk_nodes=64),
+    dict(block_threads=256, chunk_nodes=1024),
+    dict(block_threads=128, chunk_nodes=128, max_blocks=1),
+    dict(block_threads=64, chunk_nodes=64, max_blocks=3),
+    dict(block_threads=256, chunk_nodes=512, blocks_per_cu=1),
+])
+def test_launch_shape_independent(ctx, oracle_mod, tuning):
+    n, d, p, seed = 200_000, 8, 0.2, 99
+    T = oracle_mod.p_threshold(p)
+    want, _, _ = oracle_mod.generate(n, d, T, seed)
+    try:
+        ctx.set_tuning(watchdog_ms=20000, **tuning)
+        got, _ = _gpu(ctx, n, d, T, seed)
+    finally:
+        ctx.set_tuning(watchdog_ms=20000)
+    assert np.array_equal(got, want)
+
+
+def test_degree_histogram(ctx, oracle_mod):
+    n, d, p, seed = 300_000, 6, 0.4, 5
+    T = oracle_mod.p_threshold(p)
+    G, _, _ = oracle_mod.generate(n, d, T, seed)
+    deg_want, _, mx_want = oracle_mod.degrees(n, d, G)
+    hist_want = np.bincount(deg_want.astype(np.int64))
+    import cuppa
+    g = ctx.generate(n, d, p_thresh=T, seed=seed)
+    hist, mx, deg = ctx.degree_histogram(g, want_degree=True)
+    assert mx == mx_want
+    assert np.array_equal(deg.cpu().numpy().astype(np.uint64), deg_want)
+    assert np.array_equal(hist.cpu().numpy(), hist_want)
