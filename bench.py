@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X copy-model generator (one JSON line on stdout).
+
+Workload: the single-graph copy model at n = 2^30 nodes, d = 8 slots per node,
+p = 0.2, Philox seed 11 (34.4 GB of int32 targets plus 8.6 GB of int64 row
+offsets, resident in HBM). One step = one ppa_generate call (sentinel fill +
+persistent generation kernel) into the same device buffers. The metric is
+edges/s = n*d / step time, with the generation kernel's HBM-roofline fraction
+measured live with HIP events recorded on the launch stream.
+
+The CPU baseline runs the reference's own serial copy model
+(src/PA/SPA/SerialCopyModel3.cpp, built by `make -C oracle ref`) on one core
+over a bounded prefix of the same parameters.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cuppa-public_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E vendor figure
+
+WORKLOADS = {
+    # name: (n, d, p, seed)
+    "cfg3": (1 << 30, 8, 0.2, 11),
+    "cfg2": (1 << 28, 8, 0.5, 7),
+    "cfg1": (1_000_000, 4, 0.5, 42),
+}
+
+
+def algorithmic_bytes_per_slot(p: float, d: int) -> dict:
+    """Bytes per generated slot (edge), per kernel and for the whole call."""
+    gen = 4.0 + 4.0 * (1.0 - p)       # int32 target write + copy-source read
+    row_ptr = 8.0 / d                 # int64 row offset write (fill kernel)
+    return {"generate_kernel": gen, "call": gen + row_ptr}
+
+
+def read_traffic(profile_dir: str, workload: str):
+    """Measured HBM bytes per launch of the generation kernel, from the
+    committed rocprofv3 --pmc summary for this workload (None if absent)."""
+    path = os.path.join(profile_dir, f"generate_kernel_{workload}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            m = re.match(r"\s*traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", line)
+            if m:
+                return float(m.group(1))
+    return None
+
+
+def cpu_baseline(n: int, d: int, p: float, prefix: int):
+    """Reference serial copy model (one core) over the first `prefix` nodes."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "scm_ref")
+    sample = f"reference SerialCopyModel (MT19937-64), first {prefix} nodes of n={n}, d={d}, p={p}"
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as tmp:
+            cmd = [ref, str(prefix), str(d), repr(p), tmp + "/", "0"]
+            try:
+                cmd = ["taskset", "-c", "0"] + cmd if _has("taskset") else cmd
+                out = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=300).stdout
+                secs = float(re.search(r"Computation Time\s*:\s*([0-9.eE+-]+)", out).group(1))
+                return {"value": prefix * d / secs, "unit": "edges/s", "cores": 1, "kind": "reference",
+                        "sample": sample, "seconds": secs}
+            except Exception as e:  # fall through to the port
+                sample += f" (reference run failed: {e})"
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    T = oracle.p_threshold(p)
+    t0 = time.perf_counter()
+    oracle.generate(prefix, d, T, 11)
+    secs = time.perf_counter() - t0
+    return {"value": prefix * d / secs, "unit": "edges/s", "cores": 1, "kind": "port",
+            "sample": f"oracle restatement, first {prefix} nodes of n={n}, d={d}, p={p}", "seconds": secs}
+
+
+def _has(tool: str) -> bool:
+    from shutil import which
+    return which(tool) is not None
+
+
+def copy_bandwidth_gbs(torch, nbytes: int = 1 << 32) -> float:
+    a = torch.empty(nbytes // 4, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(5):
+        b.copy_(a)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / 5
+    del a, b
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg3", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-prefix", type=int, default=1 << 24,
+                    help="nodes of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--degree", action="store_true", help="also time the degree-histogram pass")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB three times (counter calibration) and exit")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: only --gpus 1 is supported; this generator runs on one GPU "
+              f"(asked for {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import cuppa
+
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        a = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 3}))
+        return
+
+    n, d, p, seed = WORKLOADS[args.workload]
+    ctx = cuppa.Context(0)
+    desc = cuppa.make_desc(n, d, p, seed)
+    targets = torch.empty(n * d, dtype=torch.int32, device="cuda")
+    row_ptr = torch.empty(n + 1, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        ctx.generate_into(desc, targets, row_ptr, stream)
+    torch.cuda.synchronize()
+
+    kernel_ms, call_ms, stats = [], [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.generate_into(desc, targets, row_ptr, stream)  # waits for its stream
+        st = ctx.stats()
+        kernel_ms.append(st["kernel_ms"])
+        call_ms.append(st["total_ms"])
+        stats.append(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    ms_per_step = wall * 1e3 / args.steps
+    edges = n * d
+    value = edges / (ms_per_step * 1e-3)
+    bpe = algorithmic_bytes_per_slot(p, d)
+    k_ms = sum(kernel_ms) / len(kernel_ms)
+    c_ms = sum(call_ms) / len(call_ms)
+    achieved = bpe["generate_kernel"] * edges / (k_ms * 1e-3) / 1e9
+    traffic = read_traffic(os.path.join(ROOT, "profiles"), args.workload)
+    last = stats[-1]
+
+    extra = {}
+    if args.degree:
+        g = cuppa.Graph(n, d, targets, row_ptr)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        hist, mx, _ = ctx.degree_histogram(g, hist_len=1 << 26)
+        torch.cuda.synchronize()
+        extra["degree_pass_ms"] = (time.perf_counter() - t1) * 1e3
+        extra["max_degree"] = mx
+    copy_gbs = copy_bandwidth_gbs(torch)
+
+    del targets, row_ptr
+    torch.cuda.empty_cache()
+    cpu = cpu_baseline(n, d, p, args.cpu_prefix) if args.cpu_prefix else None
+
+    line = {
+        "metric": "edges/s at n=2^30 d=8 + HBM-roofline %",
+        "value": value,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (keyed Philox draws)",
+        "config": {"workload": f"copy-model single graph n={n} d={d} p={p} seed={seed}",
+                   "n": n, "d": d, "p": p, "seed": seed, "edges": edges},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"generate_kernel<{d}>",
+                     "algorithmic_bytes_per_edge": bpe["generate_kernel"],
+                     "kernel_ms": k_ms},
+        "cpu_baseline": cpu,
+        "call_roofline_frac": bpe["call"] * edges / (c_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "call_ms": c_ms,
+        "measured_copy_gbs": copy_gbs,
+        "stats": {k: last[k] for k in ("draws", "retries", "prefetch_misses", "polls", "stalls",
+                                       "claims", "sleeps", "grid_blocks", "block_threads")},
+        **extra,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

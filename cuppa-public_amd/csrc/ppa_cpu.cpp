@@ -1,0 +1,76 @@
+// ppa_cpu.cpp -- the serial CPU copy model of the product (`-A scm`).
+//
+// Counterpart of SerialCopyModel::generateGraphCopyModelBST
+// (src/PA/SPA/SerialCopyModel3.cpp:11-107): one host thread walks the nodes in
+// order and fills each node's slots with redraw-until-unique. It consumes the
+// same keyed Philox draws as the GPU generator (ppa_model.hpp), so the two
+// produce identical graphs; the duplicate set is the row itself (d <= 16)
+// instead of the reference's BSTVertex (BSTVertex.h:189-235).
+#include <cstring>
+
+#include "../../include/ppa.h"
+#include "ppa_model.hpp"
+
+extern "C" {
+
+int ppa_cpu_generate(const ppa_graph_desc *desc, int32_t *targets, int64_t *row_ptr, uint64_t *retries_out)
+{
+    int rc = ppa_check_desc(desc);
+    if (rc) return rc;
+    if (!targets) return PPA_ERR_INVALID;
+    const uint64_t n = desc->n;
+    const uint32_t d = desc->d;
+    const uint64_t T = desc->p_threshold;
+    const uint32_t k0 = (uint32_t)desc->seed, k1 = (uint32_t)(desc->seed >> 32);
+    for (uint64_t i = 0; i <= d; ++i)
+        for (uint32_t j = 0; j < d; ++j) targets[i * d + j] = ppa::clique_target(i, j);
+    uint64_t retries = 0;
+    for (uint64_t t = (uint64_t)d + 1; t < n; ++t) {
+        int32_t *row = targets + t * d;
+        const uint64_t td = t * d;
+        for (uint32_t k = 0; k < d; ++k) {
+            for (uint32_t r = 0;; ++r) {
+                if (r >= PPA_MAX_RETRY) return PPA_ERR_RETRY;
+                const ppa::Attempt a = ppa::attempt(k0, k1, T, t, k, r, td);
+                const int32_t cand = a.direct ? (int32_t)(a.pair / d) : targets[a.pair];
+                bool dup = false;
+                for (uint32_t i = 0; i < k && !dup; ++i) dup = row[i] == cand;
+                if (!dup) {
+                    row[k] = cand;
+                    break;
+                }
+                ++retries;
+            }
+        }
+    }
+    if (row_ptr)
+        for (uint64_t t = 0; t <= n; ++t) row_ptr[t] = (int64_t)(t * d);
+    if (retries_out) *retries_out = retries;
+    return PPA_OK;
+}
+
+int ppa_cpu_degree_histogram(const ppa_graph_desc *desc, const int32_t *targets, uint32_t *degree,
+                             uint64_t *hist, uint64_t hist_len, uint64_t *max_degree)
+{
+    int rc = ppa_check_desc(desc);
+    if (rc) return rc;
+    if (!targets || !degree || (hist_len && !hist)) return PPA_ERR_INVALID;
+    const uint64_t n = desc->n;
+    const uint32_t d = desc->d;
+    for (uint64_t v = 0; v < n; ++v) degree[v] = d;
+    for (uint64_t s = (uint64_t)(d + 1) * d; s < n * d; ++s) {
+        const int32_t v = targets[s];
+        if (v < 0 || (uint64_t)v >= n) return PPA_ERR_INVALID;
+        degree[v]++;
+    }
+    if (hist_len) std::memset(hist, 0, hist_len * sizeof(uint64_t));
+    uint64_t mx = 0;
+    for (uint64_t v = 0; v < n; ++v) {
+        if (degree[v] > mx) mx = degree[v];
+        if (degree[v] < hist_len) hist[degree[v]]++;
+    }
+    if (max_degree) *max_degree = mx;
+    return PPA_OK;
+}
+
+}  // extern "C"

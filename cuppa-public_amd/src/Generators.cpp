@@ -1,0 +1,185 @@
+// Generators.cpp -- cuPPA and SerialCopyModel over the C ABI.
+#include "Generators.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+
+std::string joinPath(const std::string &dir, const std::string &file)
+{
+    if (dir.empty()) return file;
+    return dir.back() == '/' ? dir + file : dir + "/" + file;
+}
+
+bool writeDegreeCsv(const std::string &path, const std::vector<uint64_t> &hist)
+{
+    std::ofstream out(path);
+    if (!out) return false;
+    out << "Degree,Count\n";
+    for (size_t deg = 0; deg < hist.size(); ++deg)
+        if (hist[deg]) out << deg << "," << hist[deg] << "\n";
+    return bool(out);
+}
+
+bool writeEdgeList(const std::string &path, const int32_t *G, uint64_t n, uint32_t d)
+{
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) return false;
+    std::vector<char> buf(1 << 20);
+    std::setvbuf(f, buf.data(), _IOFBF, buf.size());
+    for (uint64_t i = 1; i <= d && i < n; ++i)
+        for (uint32_t j = 0; j < d; ++j)
+            if ((uint64_t)G[i * d + j] < i) std::fprintf(f, "%llu %d\n", (unsigned long long)i, G[i * d + j]);
+    for (uint64_t t = (uint64_t)d + 1; t < n; ++t)
+        for (uint32_t k = 0; k < d; ++k) std::fprintf(f, "%llu %d\n", (unsigned long long)t, G[t * d + k]);
+    const bool ok = std::ferror(f) == 0;
+    return std::fclose(f) == 0 && ok;
+}
+
+// ---- cuPPA ------------------------------------------------------------------
+
+cuPPA::cuPPA(int64_t nNodes, uint32_t mEdgesPerNode, double p) : _N(nNodes), _M(mEdgesPerNode), _p(p) {}
+
+cuPPA::~cuPPA()
+{
+    if (_dGraph) (void)hipFree(_dGraph);
+    if (_dRowPtr) (void)hipFree(_dRowPtr);
+    if (_ctx) ppa_ctx_destroy(_ctx);
+}
+
+ppa_graph_desc cuPPA::desc(const ExperimentConfiguration &conf) const
+{
+    ppa_graph_desc g{};
+    g.n = (uint64_t)_N;
+    g.d = _M;
+    g.p_threshold = ppa_p_threshold(_p);
+    g.seed = conf._seed;
+    return g;
+}
+
+int cuPPA::generateGraph(const ExperimentConfiguration &conf) { return run(conf, "StepwiseTiming-Original.csv"); }
+
+int cuPPA::generateGraphAdaptive(const ExperimentConfiguration &conf)
+{
+    return run(conf, "StepwiseTiming-Adaptive.csv");
+}
+
+int cuPPA::run(const ExperimentConfiguration &conf, const char *timingCsv)
+{
+    const ppa_graph_desc g = desc(conf);
+    int rc = ppa_check_desc(&g);
+    if (rc) return rc;
+    if (!_ctx && (rc = ppa_ctx_create(&_ctx, conf._device)) != PPA_OK) return rc;
+    ppa_tuning tu{};
+    if (conf._threadsGiven) {
+        const int threads = conf._threadsPerBlock < 64 ? 64 : conf._threadsPerBlock;
+        tu.block_threads = (uint32_t)((threads + 63) / 64 * 64);
+        if (tu.block_threads > 256) tu.block_threads = 256;
+    }
+    if (conf._blocksGiven && conf._blockSize > 0) tu.max_blocks = (uint32_t)conf._blockSize;
+    tu.chunk_nodes = conf._chunk;
+    if ((rc = ppa_ctx_set_tuning(_ctx, &tu)) != PPA_OK) return rc;
+
+    if (!_dGraph) {
+        if (hipMalloc(&_dGraph, g.n * g.d * sizeof(int32_t)) != hipSuccess) return PPA_ERR_NOMEM;
+        if (hipMalloc(&_dRowPtr, (g.n + 1) * sizeof(int64_t)) != hipSuccess) return PPA_ERR_NOMEM;
+    }
+    std::printf("--------------------------------------------------------------------------------\n");
+    std::printf("Graph: %.2f MB\t| RowPtr: %.2f MB\n", g.n * g.d * 4.0 / 1048576.0, (g.n + 1) * 8.0 / 1048576.0);
+    rc = ppa_generate(_ctx, &g, _dGraph, _dRowPtr, nullptr);
+    ppa_stats st{};
+    ppa_get_stats(_ctx, &st);
+    if (rc) return rc;
+    std::printf("BLOCKS: %llu\t| THREADS PER BLOCK: %llu\t| DRAWS: %llu\t| RETRIES: %llu\t| PARKED: %llu\n",
+                (unsigned long long)st.grid_blocks, (unsigned long long)st.block_threads,
+                (unsigned long long)st.draws, (unsigned long long)st.retries,
+                (unsigned long long)st.prefetch_misses);
+    {
+        // cuPPA_1.cu:84-87,217-224: one row per stage; here a single stage.
+        std::ofstream csv(timingCsv);
+        csv << "Round,nBlocks,nThreads,Edges,MaxQ,Initialization Time,Computation Time\n";
+        if (conf._N_STAGE_TIMINGS > 0)
+            csv << 1 << "," << st.grid_blocks << "," << st.block_threads << "," << g.n * g.d << ","
+                << st.prefetch_misses << "," << (st.total_ms - st.kernel_ms) * 1e-3 << ","
+                << st.kernel_ms * 1e-3 << "\n";
+    }
+    std::printf("Computation Time: %f\n\n\n", st.total_ms * 1e-3);
+    if (conf._FLAG_SHOW_DEGREE_DISTRIBUTION && (rc = printDegreeDistribution(conf)) != PPA_OK) return rc;
+    if (!conf._edgesFile.empty()) {
+        std::vector<int32_t> host(g.n * g.d);
+        if (hipMemcpy(host.data(), _dGraph, host.size() * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+            return PPA_ERR_DEVICE;
+        if (!writeEdgeList(conf._edgesFile, host.data(), g.n, g.d)) return PPA_ERR_INVALID;
+    }
+    std::cout << "Ended" << std::endl;
+    return PPA_OK;
+}
+
+int cuPPA::printDegreeDistribution(const ExperimentConfiguration &conf)
+{
+    if (!_dGraph || !_ctx) return PPA_ERR_INVALID;
+    std::printf("Generating Degree Distribution\n");
+    const ppa_graph_desc g = desc(conf);
+    uint32_t *dDeg = nullptr;
+    uint64_t *dMax = nullptr, *dHist = nullptr;
+    int rc = PPA_OK;
+    if (hipMalloc(&dDeg, g.n * sizeof(uint32_t)) != hipSuccess || hipMalloc(&dMax, sizeof(uint64_t)) != hipSuccess)
+        rc = PPA_ERR_NOMEM;
+    uint64_t mx = 0;
+    if (!rc) rc = ppa_degree_histogram(_ctx, &g, _dGraph, dDeg, nullptr, 0, dMax, nullptr);
+    if (!rc && hipMemcpy(&mx, dMax, sizeof(mx), hipMemcpyDeviceToHost) != hipSuccess) rc = PPA_ERR_DEVICE;
+    std::vector<uint64_t> hist(mx + 1, 0);
+    if (!rc && hipMalloc(&dHist, hist.size() * sizeof(uint64_t)) != hipSuccess) rc = PPA_ERR_NOMEM;
+    if (!rc) rc = ppa_degree_histogram(_ctx, &g, _dGraph, dDeg, dHist, hist.size(), dMax, nullptr);
+    if (!rc && hipMemcpy(hist.data(), dHist, hist.size() * sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = PPA_ERR_DEVICE;
+    if (!rc) {
+        // the reference prints the degrees of its seed nodes (cuPPA.cu:136-139)
+        std::vector<uint32_t> seedDeg(_M);
+        if (hipMemcpy(seedDeg.data(), dDeg, _M * sizeof(uint32_t), hipMemcpyDeviceToHost) == hipSuccess)
+            for (uint32_t u = 0; u < _M; ++u) std::cout << u << "\t" << seedDeg[u] << std::endl;
+        if (!writeDegreeCsv(joinPath(conf._outputDir, conf._distfile1), hist)) rc = PPA_ERR_INVALID;
+    }
+    if (dDeg) (void)hipFree(dDeg);
+    if (dMax) (void)hipFree(dMax);
+    if (dHist) (void)hipFree(dHist);
+    return rc;
+}
+
+// ---- SerialCopyModel ----------------------------------------------------------
+
+int SerialCopyModel::generateGraph(const ExperimentConfiguration &conf)
+{
+    const std::string METHOD = "SCM";
+    std::cout << METHOD << std::endl;
+    std::ofstream log(conf._outputDir + METHOD + ".log");  // SerialCopyModel3.cpp:19-25
+    log << METHOD << std::endl << _N << std::endl << _M << std::endl;
+    ppa_graph_desc g{};
+    g.n = (uint64_t)_N;
+    g.d = _M;
+    g.p_threshold = ppa_p_threshold(_p);
+    g.seed = conf._seed;
+    int rc = ppa_check_desc(&g);
+    if (rc) return rc;
+    _G.assign(g.n * g.d, -1);
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = ppa_cpu_generate(&g, _G.data(), nullptr, nullptr);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc) return rc;
+    std::cout << "Computation Time : " << secs << std::endl;
+    log << "Computation Time : " << secs << std::endl;
+    if (conf._FLAG_SHOW_DEGREE_DISTRIBUTION) {
+        std::vector<uint32_t> deg(g.n);
+        uint64_t mx = 0;
+        if ((rc = ppa_cpu_degree_histogram(&g, _G.data(), deg.data(), nullptr, 0, &mx)) != PPA_OK) return rc;
+        std::vector<uint64_t> hist(mx + 1, 0);
+        if ((rc = ppa_cpu_degree_histogram(&g, _G.data(), deg.data(), hist.data(), hist.size(), &mx)) != PPA_OK)
+            return rc;
+        if (!writeDegreeCsv(joinPath(conf._outputDir, conf._distfile1), hist)) return PPA_ERR_INVALID;
+    }
+    if (!conf._edgesFile.empty() && !writeEdgeList(conf._edgesFile, _G.data(), g.n, g.d)) return PPA_ERR_INVALID;
+    return PPA_OK;
+}

@@ -1,0 +1,68 @@
+// Generators.hpp -- host-side generator objects of the driver.
+//
+// cuPPA mirrors the reference class of src/PA/cuPPA/cuPPA.cuh:84-105 (same
+// constructor arguments and entry points) over the C ABI of include/ppa.h;
+// SerialCopyModel mirrors src/PA/SPA/SerialCopyModel.h:26-47 over
+// ppa_cpu_generate. Unlike the reference, a generator keeps its graph until
+// it is destroyed and reports failures as a nonzero return value.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ExperimentConfiguration.hpp"
+#include "ppa.h"
+
+class cuPPA {
+public:
+    cuPPA(int64_t nNodes, uint32_t mEdgesPerNode, double p);
+    ~cuPPA();
+    cuPPA(const cuPPA &) = delete;
+    cuPPA &operator=(const cuPPA &) = delete;
+
+    // cuPPA_1.cu:9-254. Returns 0 or a PPA_ERR_* code.
+    int generateGraph(const ExperimentConfiguration &conf);
+    // cuPPA_2.cu:9-241: the reference grows threads and stage size online
+    // while its queues stay half empty; here one persistent launch already
+    // spans every stage, so this runs the same generator and records the
+    // run under the adaptive CSV name.
+    int generateGraphAdaptive(const ExperimentConfiguration &conf);
+    // cuPPA.cu:103-165
+    int printDegreeDistribution(const ExperimentConfiguration &conf);
+
+    const int32_t *deviceTargets() const { return _dGraph; }
+
+private:
+    int run(const ExperimentConfiguration &conf, const char *timingCsv);
+    ppa_graph_desc desc(const ExperimentConfiguration &conf) const;
+
+    int64_t _N;
+    uint32_t _M;
+    double _p;
+    int32_t *_dGraph = nullptr;
+    int64_t *_dRowPtr = nullptr;
+    ppa_ctx *_ctx = nullptr;
+};
+
+class SerialCopyModel {
+public:
+    SerialCopyModel(int64_t n, uint32_t m, double p) : _N(n), _M(m), _p(p) {}
+    // SerialCopyModel3.cpp:11-107
+    int generateGraph(const ExperimentConfiguration &conf);
+    const std::vector<int32_t> &targets() const { return _G; }
+
+private:
+    int64_t _N;
+    uint32_t _M;
+    double _p;
+    std::vector<int32_t> _G;
+};
+
+// Output formats of the reference.
+// "Degree,Count" CSV of cuPPA.cu:148-160 and PAHelper.hpp:54-63.
+bool writeDegreeCsv(const std::string &path, const std::vector<uint64_t> &hist);
+// "u v" edge list of PAHelper::writeGraphToDisk (PAHelper.hpp:163-177): the
+// seed clique's edges once each, then every slot of nodes t >= d+1.
+bool writeEdgeList(const std::string &path, const int32_t *targets, uint64_t n, uint32_t d);
+std::string joinPath(const std::string &dir, const std::string &file);

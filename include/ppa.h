@@ -136,6 +136,17 @@ int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t
                          uint32_t *d_degree, uint64_t *d_hist, uint64_t hist_len,
                          uint64_t *d_max_degree, void *stream);
 
+/* ---- serial CPU path (`-A scm`): replaces SerialCopyModel::generateGraph
+ *      (src/PA/SPA/SerialCopyModel3.cpp:11-107) ---------------------------- */
+/* Host memory. Same model and draws as ppa_generate, hence the same graph.
+ * retries (nullable) receives the number of rejected attempts. */
+int ppa_cpu_generate(const ppa_graph_desc *desc, int32_t *targets, int64_t *row_ptr, uint64_t *retries);
+/* Host-memory degree pass of PAHelper::outputSCMDegreeDistribution
+ * (PAHelper.hpp:15-67) with deg(v) as in ppa_degree_histogram. degree is
+ * uint32[n] scratch supplied by the caller. */
+int ppa_cpu_degree_histogram(const ppa_graph_desc *desc, const int32_t *targets, uint32_t *degree,
+                             uint64_t *hist, uint64_t hist_len, uint64_t *max_degree);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,8 @@ def lib():
     L.oracle_generate.restype = ctypes.c_int
     L.oracle_closure_create.argtypes = [u64, u32, u64, u64]
     L.oracle_closure_create.restype = ctypes.c_void_p
+    L.oracle_closure_set_prefix.argtypes = [ctypes.c_void_p, i32p, u64]
+    L.oracle_closure_set_prefix.restype = ctypes.c_int
     L.oracle_closure_destroy.argtypes = [ctypes.c_void_p]
     L.oracle_closure_destroy.restype = None
     L.oracle_closure_target.argtypes = [ctypes.c_void_p, u64, u32, i32p]
@@ -64,6 +66,11 @@ def lib():
     L.oracle_degrees.argtypes = [u64, u32, i32p, ctypes.POINTER(u64), ctypes.POINTER(u64), u64,
                                  ctypes.POINTER(u64)]
     L.oracle_degrees.restype = ctypes.c_int
+    L.oracle_verify_rows.argtypes = [u64, u32, u64, u64, i32p, u64, u64, ctypes.POINTER(u64)]
+    L.oracle_verify_rows.restype = ctypes.c_int
+    L.oracle_verify_parallel.argtypes = [u64, u32, u64, u64, i32p, u64, u64, ctypes.c_int,
+                                         ctypes.POINTER(u64)]
+    L.oracle_verify_parallel.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -108,11 +115,22 @@ def generate(n: int, d: int, T: int, seed: int, with_row_ptr: bool = False):
 class Closure:
     """Memoised target(t, k) for graphs too large to generate sequentially."""
 
-    def __init__(self, n, d, T, seed):
+    def __init__(self, n, d, T, seed, prefix: np.ndarray | None = None):
         self.n, self.d = n, d
         self.h = lib().oracle_closure_create(n, d, T, seed)
         if not self.h:
             raise ValueError("bad closure parameters")
+        self._prefix = None
+        if prefix is not None:
+            self.set_prefix(prefix)
+
+    def set_prefix(self, prefix: np.ndarray):
+        """prefix: int32 rows 0..m-1 of this graph from the sequential oracle."""
+        self._prefix = np.ascontiguousarray(prefix, dtype=np.int32)
+        m = self._prefix.size // self.d
+        rc = lib().oracle_closure_set_prefix(self.h, _ptr(self._prefix, ctypes.c_int32), m)
+        if rc != 0:
+            raise RuntimeError(f"oracle_closure_set_prefix failed: {rc}")
 
     def rows(self, first: int, count: int) -> np.ndarray:
         out = np.empty(count * self.d, dtype=np.int32)
@@ -138,6 +156,26 @@ class Closure:
 
     def __del__(self):
         self.close()
+
+
+NO_BAD_ROW = (1 << 64) - 1
+
+
+def verify_rows(n: int, d: int, T: int, seed: int, G: np.ndarray, first: int = 0, count: int | None = None,
+                threads: int = 1):
+    """Row-consistency check of a complete candidate graph G (int32 [n*d]).
+    Returns the first row in [first, first+count) that is not what the model
+    produces from G's earlier rows, or None if all pass."""
+    G = np.ascontiguousarray(G, dtype=np.int32)
+    assert G.size == n * d
+    if count is None:
+        count = n - first
+    bad = ctypes.c_uint64()
+    rc = lib().oracle_verify_parallel(n, d, T, seed, _ptr(G, ctypes.c_int32), first, count, threads,
+                                      ctypes.byref(bad))
+    if rc != 0:
+        raise RuntimeError(f"oracle_verify_parallel failed: {rc}")
+    return None if bad.value == NO_BAD_ROW else int(bad.value)
 
 
 def degrees(n: int, d: int, G: np.ndarray, hist_len: int = 0):

@@ -32,6 +32,7 @@
  * special cases (p = 1, p = 0, d = 1), invariants and a pure-Python
  * restatement on small inputs; the closure oracle against the sequential one.
  */
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -167,6 +168,8 @@ typedef struct {
     uint64_t used;
     uint64_t n, T, seed;
     uint32_t d;
+    const int32_t *prefix;   /* optional: rows [0, prefix_nodes) from the sequential oracle */
+    uint64_t prefix_nodes;
 } oracle_closure;
 
 static uint64_t mix64(uint64_t x)
@@ -188,6 +191,17 @@ oracle_closure *oracle_closure_create(uint64_t n, uint32_t d, uint64_t T, uint64
     if (!c->keys || !c->vals) { free(c->keys); free(c->vals); free(c); return NULL; }
     c->n = n; c->d = d; c->T = T; c->seed = seed;
     return c;
+}
+
+/* Rows below prefix_nodes are read from `prefix` (rows 0 .. prefix_nodes-1 of
+ * the same graph, produced by oracle_generate) instead of being recursed into.
+ * The array must outlive the handle. */
+int oracle_closure_set_prefix(oracle_closure *c, const int32_t *prefix, uint64_t prefix_nodes)
+{
+    if (!c || (prefix_nodes && !prefix) || prefix_nodes > c->n) return ORACLE_ERR_ARG;
+    c->prefix = prefix;
+    c->prefix_nodes = prefix_nodes;
+    return ORACLE_OK;
 }
 
 void oracle_closure_destroy(oracle_closure *c)
@@ -237,6 +251,7 @@ int oracle_closure_target(oracle_closure *c, uint64_t t, uint32_t k, int32_t *ou
     if (t >= c->n || k >= d) return ORACLE_ERR_ARG;
     if (t <= d) { *out = (int32_t)(k + (k >= t)); return ORACLE_OK; }
     uint64_t s = t * d + k;
+    if (t < c->prefix_nodes) { *out = c->prefix[s]; return ORACLE_OK; }
     if (closure_find(c, s, out)) return ORACLE_OK;
     int32_t prev[64];
     int32_t *row = prev;
@@ -308,4 +323,94 @@ int oracle_degrees(uint64_t n, uint32_t d, const int32_t *G, uint64_t *deg_out,
     if (max_degree) *max_degree = mx;
     if (!deg_out) free(deg);
     return ORACLE_OK;
+}
+
+/* ---- row-consistency verifier --------------------------------------------- */
+/*
+ * Checks rows [first, first + count) of a complete candidate graph G: each row
+ * must equal what the model produces for node t when its copy reads are served
+ * from G's own earlier rows (SerialCopyModel3.cpp:57-77 for one node). Rows
+ * 0..d must be the seed clique. If every row of a graph passes, the graph
+ * equals oracle_generate's output by induction over t (the first row that
+ * differs from the oracle would have all of its copy sources right and so
+ * could not pass). Returns the first failing row through *bad (or UINT64_MAX).
+ */
+int oracle_verify_rows(uint64_t n, uint32_t d, uint64_t T, uint64_t seed, const int32_t *G,
+                       uint64_t first, uint64_t count, uint64_t *bad)
+{
+    *bad = UINT64_MAX;
+    if (d == 0 || d > 4096 || n < (uint64_t)d + 1 || first + count > n) return ORACLE_ERR_ARG;
+    int32_t row[4096];
+    for (uint64_t t = first; t < first + count; ++t) {
+        const int32_t *g = G + t * d;
+        if (t <= d) {
+            for (uint32_t k = 0; k < d; ++k)
+                if (g[k] != (int32_t)(k + (k >= t))) { *bad = t; return ORACLE_OK; }
+            continue;
+        }
+        for (uint32_t k = 0; k < d; ++k) {
+            uint32_t r = 0;
+            int32_t cand;
+            for (;;) {
+                int direct;
+                uint64_t u;
+                uint32_t j;
+                oracle_attempt(seed, T, t, d, k, r, &direct, &u, &j);
+                cand = direct ? (int32_t)u : G[u * d + j];
+                int dup = 0;
+                for (uint32_t i = 0; i < k; ++i)
+                    if (row[i] == cand) { dup = 1; break; }
+                if (!dup) break;
+                if (++r >= ORACLE_MAX_RETRY) { *bad = t; return ORACLE_OK; }
+            }
+            if (cand != g[k]) { *bad = t; return ORACLE_OK; }
+            row[k] = cand;
+        }
+    }
+    return ORACLE_OK;
+}
+
+typedef struct {
+    uint64_t n, T, seed, first, count, bad;
+    uint32_t d;
+    const int32_t *G;
+    int rc;
+} verify_job;
+
+static void *verify_worker(void *arg)
+{
+    verify_job *j = (verify_job *)arg;
+    j->rc = oracle_verify_rows(j->n, j->d, j->T, j->seed, j->G, j->first, j->count, &j->bad);
+    return NULL;
+}
+
+/* oracle_verify_rows over [first, first+count) split across `threads` threads. */
+int oracle_verify_parallel(uint64_t n, uint32_t d, uint64_t T, uint64_t seed, const int32_t *G,
+                           uint64_t first, uint64_t count, int threads, uint64_t *bad)
+{
+    *bad = UINT64_MAX;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    verify_job jobs[256];
+    pthread_t tid[256];
+    uint64_t per = (count + threads - 1) / threads;
+    int started = 0;
+    for (int i = 0; i < threads; ++i) {
+        uint64_t lo = first + (uint64_t)i * per;
+        if (lo >= first + count) break;
+        uint64_t hi = lo + per < first + count ? lo + per : first + count;
+        jobs[i] = (verify_job){n, T, seed, lo, hi - lo, UINT64_MAX, d, G, 0};
+        if (pthread_create(&tid[i], NULL, verify_worker, &jobs[i]) != 0) {
+            verify_worker(&jobs[i]);
+            tid[i] = 0;
+        }
+        started = i + 1;
+    }
+    int rc = ORACLE_OK;
+    for (int i = 0; i < started; ++i) {
+        if (tid[i]) pthread_join(tid[i], NULL);
+        if (jobs[i].rc != ORACLE_OK) rc = jobs[i].rc;
+        if (jobs[i].bad < *bad) *bad = jobs[i].bad;
+    }
+    return rc;
 }

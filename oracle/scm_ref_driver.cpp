@@ -9,7 +9,7 @@
 // the default seed (Random64.hpp:75-76), so its output is compared with ours
 // statistically, not bit for bit.
 //
-// usage: scm_ref <n> <m> <p> <outdir/>
+// usage: scm_ref <n> <m> <p> <outdir/> [write_dd=1]
 #include <cstdlib>
 #include <iostream>
 
@@ -17,8 +17,8 @@
 
 int main(int argc, char **argv)
 {
-    if (argc != 5) {
-        std::cerr << "usage: scm_ref <n> <m> <p> <outdir/>" << std::endl;
+    if (argc != 5 && argc != 6) {
+        std::cerr << "usage: scm_ref <n> <m> <p> <outdir/> [write_dd=1]" << std::endl;
         return 2;
     }
     ExperimentConfiguration conf;
@@ -27,7 +27,7 @@ int main(int argc, char **argv)
     conf._alpha = std::atof(argv[3]);
     conf._outputDir = argv[4];
     conf._distfile1 = "scm_dd.csv";
-    conf._FLAG_SHOW_DEGREE_DISTRIBUTION = true;
+    conf._FLAG_SHOW_DEGREE_DISTRIBUTION = argc == 6 ? std::atoi(argv[5]) != 0 : true;
     conf._N_STAGE_TIMINGS = 0;
     SerialCopyModel scm(conf._n1, (EdgeIndexType)conf._x1, conf._alpha);
     scm.generateGraph(conf);
