@@ -6,110 +6,75 @@
 // loop (cuPPA_1.cu:112-226) becomes one persistent launch, and the
 // printf-and-continue error handling (cuPPA_1.cu:42-46,106-110) becomes
 // status codes.
-#include <hip/hip_runtime.h>
-
 #include <cstring>
 #include <new>
 
-#include "../../include/ppa.h"
-#include "ppa_kernels.hpp"
+#include "ppa_internal.hpp"
 
-struct ppa_ctx {
-    int device = 0;
-    int num_cus = 0;
-    int wall_clock_khz = 0;
-    ppa_tuning tuning{};
-    ppa::DeviceScratch *scratch = nullptr;  // device
-    uint32_t *degree_scratch = nullptr;     // device, grown on demand
-    uint64_t degree_scratch_len = 0;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    ppa_stats last{};
-    bool pending = false;  // an async generate awaits ppa_ctx_sync
-    uint64_t pending_n = 0;
-    uint32_t pending_d = 0;
-};
+namespace ppa {
 
-namespace {
+bool tuning_valid(const ppa_tuning &t)
+{
+    if (t.block_threads && (t.block_threads % 64 != 0 || t.block_threads > 256)) return false;
+    if (t.chunk_nodes && (t.chunk_nodes % 64 != 0 || t.chunk_nodes > (1u << 20))) return false;
+    return true;
+}
 
-constexpr uint32_t kDefaultBlock = 256;
-constexpr uint32_t kDefaultChunk = 256;
-constexpr uint32_t kDefaultWatchdogMs = 60000;
-
-class DeviceGuard {
-public:
-    explicit DeviceGuard(int dev)
-    {
-        if (hipGetDevice(&prev_) != hipSuccess) prev_ = -1;
-        ok_ = hipSetDevice(dev) == hipSuccess;
-    }
-    ~DeviceGuard()
-    {
-        if (prev_ >= 0) (void)hipSetDevice(prev_);
-    }
-    bool ok() const { return ok_; }
-
-private:
-    int prev_ = -1;
-    bool ok_ = false;
-};
-
-#define PPA_HIP(call)                               \
-    do {                                            \
-        if ((call) != hipSuccess) return PPA_ERR_DEVICE; \
-    } while (0)
-
-int resolve_tuning(const ppa_ctx *ctx, uint32_t d, uint64_t nodes, uint32_t *blocks, uint32_t *threads,
-                   uint32_t *chunk)
+int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *out)
 {
     const ppa_tuning &tu = ctx->tuning;
-    uint32_t th = tu.block_threads ? tu.block_threads : kDefaultBlock;
-    if (th % 64 != 0 || th > 256) return PPA_ERR_INVALID;  // kernels are built for <= 256
-    uint32_t ch = tu.chunk_nodes ? tu.chunk_nodes : kDefaultChunk;
-    if (ch % 64 != 0) return PPA_ERR_INVALID;
+    if (!tuning_valid(tu)) return PPA_ERR_INVALID;
+    const uint32_t th = tu.block_threads ? tu.block_threads : kDefaultBlock;
+    const uint32_t ch = tu.chunk_nodes ? tu.chunk_nodes : kDefaultChunk;
     int per_cu = 0;
-    PPA_HIP(ppa::generate_occupancy(d, th, &per_cu));
+    PPA_HIP(generate_occupancy(d, th, &per_cu));
     if (per_cu < 1) per_cu = 1;
     if (tu.blocks_per_cu && (int)tu.blocks_per_cu < per_cu) per_cu = (int)tu.blocks_per_cu;
     uint64_t g = (uint64_t)per_cu * (uint64_t)ctx->num_cus;
     // no more wavefronts than there are chunks of work
-    const uint64_t waves_needed = (nodes + ch - 1) / ch;
+    const uint64_t waves_needed = (work + ch - 1) / ch;
     const uint64_t blocks_needed = (waves_needed + th / 64 - 1) / (th / 64);
     if (g > blocks_needed) g = blocks_needed;
     if (tu.max_blocks && g > tu.max_blocks) g = tu.max_blocks;
     if (g < 1) g = 1;
-    *blocks = (uint32_t)g;
-    *threads = th;
-    *chunk = ch;
+    out->blocks = (uint32_t)g;
+    out->threads = th;
+    out->chunk = ch;
+    out->early_nodes = tu.early_nodes ? tu.early_nodes : kDefaultEarlyNodes;
+    const uint32_t wd = tu.watchdog_ms ? tu.watchdog_ms : kDefaultWatchdogMs;
+    out->watchdog_ticks = (uint64_t)wd * (uint64_t)ctx->wall_clock_khz;
     return PPA_OK;
 }
 
-int finish(ppa_ctx *ctx, hipStream_t stream)
+int finish_launch(ppa_ctx *ctx, hipStream_t stream)
 {
     PPA_HIP(hipStreamSynchronize(stream));
-    ppa::DeviceScratch host{};
+    DeviceScratch host{};
     PPA_HIP(hipMemcpy(&host, ctx->scratch, sizeof(host), hipMemcpyDeviceToHost));
     ppa_stats &s = ctx->last;
-    s.nodes = host.stats[ppa::kStatNodes];
-    s.slots = host.stats[ppa::kStatSlots];
-    s.draws = host.stats[ppa::kStatDraws];
-    s.retries = host.stats[ppa::kStatRetries];
-    s.prefetch_misses = host.stats[ppa::kStatPrefetchMisses];
-    s.polls = host.stats[ppa::kStatPolls];
-    s.stalls = host.stats[ppa::kStatStalls];
-    s.claims = host.stats[ppa::kStatClaims];
-    s.sleeps = host.stats[ppa::kStatSleeps];
+    s.nodes = host.stats[kStatNodes];
+    s.slots = host.stats[kStatSlots];
+    s.draws = host.stats[kStatDraws];
+    s.retries = host.stats[kStatRetries];
+    s.prefetch_misses = host.stats[kStatPrefetchMisses];
+    s.polls = host.stats[kStatPolls];
+    s.stalls = host.stats[kStatStalls];
+    s.claims = host.stats[kStatClaims];
+    s.sleeps = host.stats[kStatSleeps];
+    s.staged_rows = host.stats[kStatStagedRows];
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) s.kernel_ms = ms;
     if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]) == hipSuccess) s.total_ms = ms;
     ctx->pending = false;
     if (host.status != 0) return host.status;
     // every generated node must have been finished by exactly one lane
-    const uint64_t expect = ctx->pending_n - (uint64_t)ctx->pending_d - 1;
-    if (s.nodes != expect) return PPA_ERR_INTERNAL;
+    if (s.nodes != ctx->expect_nodes) return PPA_ERR_INTERNAL;
     return PPA_OK;
 }
 
-}  // namespace
+}  // namespace ppa
+
+using namespace ppa;
 
 extern "C" {
 
@@ -165,7 +130,7 @@ int ppa_ctx_create(ppa_ctx **out, int device)
         return PPA_ERR_DEVICE;
     }
     if (c->wall_clock_khz <= 0) c->wall_clock_khz = 100000;
-    if (hipMalloc(&c->scratch, sizeof(ppa::DeviceScratch)) != hipSuccess) {
+    if (hipMalloc(&c->scratch, sizeof(DeviceScratch)) != hipSuccess) {
         delete c;
         return PPA_ERR_NOMEM;
     }
@@ -193,9 +158,7 @@ int ppa_ctx_destroy(ppa_ctx *ctx)
 
 int ppa_ctx_set_tuning(ppa_ctx *ctx, const ppa_tuning *t)
 {
-    if (!ctx || !t) return PPA_ERR_INVALID;
-    if (t->block_threads && (t->block_threads % 64 != 0 || t->block_threads > 256)) return PPA_ERR_INVALID;
-    if (t->chunk_nodes && t->chunk_nodes % 64 != 0) return PPA_ERR_INVALID;
+    if (!ctx || !t || !tuning_valid(*t)) return PPA_ERR_INVALID;
     ctx->tuning = *t;
     return PPA_OK;
 }
@@ -226,35 +189,34 @@ int ppa_generate_async(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targ
     hipStream_t stream = (hipStream_t)stream_;
     const uint64_t n = desc->n;
     const uint32_t d = desc->d;
-    uint32_t blocks = 0, threads = 0, chunk = 0;
-    rc = resolve_tuning(ctx, d, n - d - 1, &blocks, &threads, &chunk);
-    if (rc) return rc;
+    LaunchShape shape;
+    if ((rc = resolve_shape(ctx, d, n - d - 1, &shape)) != PPA_OK) return rc;
 
     ctx->last = ppa_stats{};
-    ctx->last.grid_blocks = blocks;
-    ctx->last.block_threads = threads;
+    ctx->last.grid_blocks = shape.blocks;
+    ctx->last.block_threads = shape.threads;
     PPA_HIP(hipEventRecord(ctx->ev[0], stream));
-    PPA_HIP(ppa::launch_reset_scratch(ctx->scratch, (uint64_t)d + 1, stream));
-    ppa::FillArgs fa{d_targets, d_row_ptr, n, d, 0};
-    PPA_HIP(ppa::launch_fill(fa, stream));
+    PPA_HIP(launch_reset_scratch(ctx->scratch, (uint64_t)d + 1, stream));
+    FillArgs fa{d_targets, d_row_ptr, n, d, 0};
+    PPA_HIP(launch_fill(fa, stream));
     PPA_HIP(hipEventRecord(ctx->ev[1], stream));
     if (n > (uint64_t)d + 1) {
-        ppa::GenArgs ga{};
+        GenArgs ga{};
         ga.G = d_targets;
         ga.n = n;
         ga.T = desc->p_threshold;
         ga.key0 = (uint32_t)desc->seed;
         ga.key1 = (uint32_t)(desc->seed >> 32);
-        ga.chunk = chunk;
-        const uint32_t wd = ctx->tuning.watchdog_ms ? ctx->tuning.watchdog_ms : kDefaultWatchdogMs;
-        ga.watchdog_ticks = (uint64_t)wd * (uint64_t)ctx->wall_clock_khz;
+        ga.chunk = shape.chunk;
+        ga.early_chunk = kEarlyChunk;
+        ga.early_nodes = shape.early_nodes;
+        ga.watchdog_ticks = shape.watchdog_ticks;
         ga.scratch = ctx->scratch;
-        PPA_HIP(ppa::launch_generate(d, ga, blocks, threads, stream));
+        PPA_HIP(launch_generate(d, ga, shape.blocks, shape.threads, stream));
     }
     PPA_HIP(hipEventRecord(ctx->ev[2], stream));
     ctx->pending = true;
-    ctx->pending_n = n;
-    ctx->pending_d = d;
+    ctx->expect_nodes = n - d - 1;
     return PPA_OK;
 }
 
@@ -264,7 +226,7 @@ int ppa_ctx_sync(ppa_ctx *ctx, void *stream)
     if (!ctx->pending) return PPA_OK;
     DeviceGuard g(ctx->device);
     if (!g.ok()) return PPA_ERR_DEVICE;
-    return finish(ctx, (hipStream_t)stream);
+    return finish_launch(ctx, (hipStream_t)stream);
 }
 
 int ppa_generate(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets, int64_t *d_row_ptr,
@@ -297,8 +259,8 @@ int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t
         }
         deg = ctx->degree_scratch;
     }
-    PPA_HIP(ppa::launch_reset_scratch(ctx->scratch, 0, stream));
-    PPA_HIP(ppa::launch_degree(d_targets, desc->n, desc->d, deg, d_hist, hist_len, ctx->scratch, stream));
+    PPA_HIP(launch_reset_scratch(ctx->scratch, 0, stream));
+    PPA_HIP(launch_degree(d_targets, desc->n, desc->d, deg, d_hist, hist_len, ctx->scratch, stream));
     if (d_max_degree)
         PPA_HIP(hipMemcpyAsync(d_max_degree, &ctx->scratch->max_degree, sizeof(uint64_t),
                                hipMemcpyDeviceToDevice, stream));

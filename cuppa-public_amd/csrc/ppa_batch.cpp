@@ -1,0 +1,329 @@
+// ppa_batch.cpp -- batched ensembles in one captured persistent launch, and
+// the autotuner.
+//
+// The reference generates one graph per process and tunes online by doubling
+// threads and stage size while its LDS queues stay half empty
+// (cuPPA_2.cu:181-198). Here a batch of graphs is one work space: graph i
+// owns a chunk-aligned range of work indices, wavefronts claim chunks of that
+// space in ascending order (a chunk never spans two graphs), and the reset,
+// fill and generation kernels are captured once into a hipGraph that every
+// launch replays. The autotuner measures launch shapes on the caller's own
+// graphs and keeps the fastest per workload signature.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ppa_internal.hpp"
+
+struct ppa_batch {
+    ppa_ctx *ctx = nullptr;
+    uint64_t count = 0;
+    void *d_tables = nullptr;  // packed as in ppa::BatchTables
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t expect_nodes = 0;
+    ppa::LaunchShape shape;
+};
+
+namespace ppa {
+
+size_t batch_tables_bytes(uint64_t count)
+{
+    return count * sizeof(BatchGraph) + 3 * (count + 1) * sizeof(uint64_t);
+}
+
+BatchGraph graph_record(const ppa_graph_desc &dsc)
+{
+    BatchGraph gr;
+    std::memset(&gr, 0, sizeof(gr));
+    gr.n = dsc.n;
+    gr.T = dsc.p_threshold;
+    gr.key0 = (uint32_t)dsc.seed;
+    gr.key1 = (uint32_t)(dsc.seed >> 32);
+    gr.d = dsc.d;
+    gr.inv_d = 1.0 / (double)dsc.d;
+    return gr;
+}
+
+void build_batch_tables(const std::vector<BatchGraph> &graphs, uint64_t chunk, BatchTables *out)
+{
+    const uint64_t count = graphs.size();
+    out->count = count;
+    out->bytes.assign(batch_tables_bytes(count), 0);
+    char *base = out->bytes.data();
+    std::memcpy(base, graphs.data(), count * sizeof(BatchGraph));
+    auto *work = (uint64_t *)(base + count * sizeof(BatchGraph));
+    auto *slot = work + (count + 1);
+    auto *row = slot + (count + 1);
+    work[0] = slot[0] = row[0] = 0;
+    out->work = 0;
+    out->any_row_ptr = false;
+    for (uint64_t i = 0; i < count; ++i) {
+        const BatchGraph &g = graphs[i];
+        const uint64_t w = g.n - g.d - 1;
+        out->work += w;
+        work[i + 1] = work[i] + (w + chunk - 1) / chunk * chunk;
+        slot[i + 1] = slot[i] + g.n * g.d;
+        row[i + 1] = row[i] + (g.n + 1);
+        out->any_row_ptr |= g.row_ptr != nullptr;
+    }
+    out->total_slots = slot[count];
+    out->total_rows = out->any_row_ptr ? row[count] : 0;
+}
+
+hipError_t enqueue_batch(void *d_tables, const BatchTables &t, const LaunchShape &shape, DeviceScratch *scratch,
+                         hipStream_t stream)
+{
+    char *base = (char *)d_tables;
+    auto *d_graphs = (BatchGraph *)base;
+    auto *d_work = (uint64_t *)(base + t.count * sizeof(BatchGraph));
+    auto *d_slot = d_work + (t.count + 1);
+    auto *d_row = d_slot + (t.count + 1);
+    hipError_t e = launch_reset_scratch(scratch, 0, stream);
+    if (e != hipSuccess) return e;
+    BatchFillArgs fa{d_graphs, d_slot, d_row, t.count};
+    if ((e = launch_fill_batch(fa, t.total_slots, t.total_rows, stream)) != hipSuccess) return e;
+    if (t.work == 0) return hipSuccess;
+    GenArgs ga{};
+    ga.graphs = d_graphs;
+    ga.work_base = d_work;
+    ga.count = t.count;
+    ga.chunk = shape.chunk;
+    ga.early_chunk = shape.chunk;
+    ga.early_nodes = shape.early_nodes;
+    ga.watchdog_ticks = shape.watchdog_ticks;
+    ga.scratch = scratch;
+    return launch_generate_batch(ga, shape.blocks, shape.threads, stream);
+}
+
+}  // namespace ppa
+
+namespace {
+
+using namespace ppa;
+
+// Graph regions [off, off + len) must not overlap.
+bool disjoint(std::vector<std::pair<uint64_t, uint64_t>> r)
+{
+    std::sort(r.begin(), r.end());
+    for (size_t i = 1; i < r.size(); ++i)
+        if (r[i].first < r[i - 1].first + r[i - 1].second) return false;
+    return true;
+}
+
+void destroy(ppa_batch *b)
+{
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->d_tables) (void)hipFree(b->d_tables);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ppa_batch_create(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, int32_t *d_targets,
+                     int64_t *d_row_ptr, ppa_batch **out)
+{
+    if (!ctx || !descs || !count || !d_targets || !out) return PPA_ERR_INVALID;
+    *out = nullptr;
+    std::vector<std::pair<uint64_t, uint64_t>> treg, rreg;
+    std::vector<BatchGraph> graphs(count);
+    for (uint64_t i = 0; i < count; ++i) {
+        int rc = ppa_check_desc(&descs[i]);
+        if (rc) return rc;
+        treg.emplace_back(descs[i].targets_offset, descs[i].n * descs[i].d);
+        rreg.emplace_back(descs[i].row_ptr_offset, descs[i].n + 1);
+        graphs[i] = graph_record(descs[i]);
+        graphs[i].G = d_targets + descs[i].targets_offset;
+        graphs[i].row_ptr = d_row_ptr ? d_row_ptr + descs[i].row_ptr_offset : nullptr;
+        graphs[i].targets_offset = descs[i].targets_offset;
+    }
+    if (!disjoint(treg) || (d_row_ptr && !disjoint(rreg))) return PPA_ERR_INVALID;
+    DeviceGuard g(ctx->device);
+    if (!g.ok()) return PPA_ERR_DEVICE;
+
+    ppa_batch *b = new (std::nothrow) ppa_batch;
+    if (!b) return PPA_ERR_NOMEM;
+    b->ctx = ctx;
+    b->count = count;
+    uint64_t work = 0;
+    for (const auto &gr : graphs) work += gr.n - gr.d - 1;
+    int rc = resolve_shape(ctx, 0, work, &b->shape);
+    if (rc) {
+        destroy(b);
+        return rc;
+    }
+    BatchTables tables;
+    build_batch_tables(graphs, b->shape.chunk, &tables);
+    b->expect_nodes = tables.work;
+    if (hipMalloc(&b->d_tables, tables.bytes.size()) != hipSuccess) {
+        destroy(b);
+        return PPA_ERR_NOMEM;
+    }
+    if (hipMemcpy(b->d_tables, tables.bytes.data(), tables.bytes.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        destroy(b);
+        return PPA_ERR_DEVICE;
+    }
+    // Capture reset + fill + generate once; launches replay the graph.
+    hipStream_t cap = nullptr;
+    if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
+        destroy(b);
+        return PPA_ERR_DEVICE;
+    }
+    bool ok = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        ok = enqueue_batch(b->d_tables, tables, b->shape, ctx->scratch, cap) == hipSuccess;
+        hipGraph_t graph = nullptr;
+        const bool ended = hipStreamEndCapture(cap, &graph) == hipSuccess;
+        ok = ok && ended;
+        b->graph = graph;
+    }
+    (void)hipStreamDestroy(cap);
+    if (!ok || hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+        destroy(b);
+        return PPA_ERR_DEVICE;
+    }
+    *out = b;
+    return PPA_OK;
+}
+
+int ppa_batch_launch(ppa_batch *b, void *stream_)
+{
+    if (!b) return PPA_ERR_INVALID;
+    ppa_ctx *ctx = b->ctx;
+    DeviceGuard g(ctx->device);
+    if (!g.ok()) return PPA_ERR_DEVICE;
+    hipStream_t stream = (hipStream_t)stream_;
+    ctx->last = ppa_stats{};
+    ctx->last.grid_blocks = b->shape.blocks;
+    ctx->last.block_threads = b->shape.threads;
+    PPA_HIP(hipEventRecord(ctx->ev[0], stream));
+    PPA_HIP(hipEventRecord(ctx->ev[1], stream));
+    PPA_HIP(hipGraphLaunch(b->exec, stream));
+    PPA_HIP(hipEventRecord(ctx->ev[2], stream));
+    ctx->pending = true;
+    ctx->expect_nodes = b->expect_nodes;
+    return PPA_OK;
+}
+
+int ppa_batch_sync(ppa_batch *b, void *stream)
+{
+    if (!b) return PPA_ERR_INVALID;
+    return ppa_ctx_sync(b->ctx, stream);
+}
+
+int ppa_batch_destroy(ppa_batch *b)
+{
+    if (!b) return PPA_ERR_INVALID;
+    DeviceGuard g(b->ctx->device);
+    destroy(b);
+    return PPA_OK;
+}
+
+int ppa_generate_batch(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, int32_t *d_targets,
+                       int64_t *d_row_ptr, void *stream)
+{
+    ppa_batch *b = nullptr;
+    int rc = ppa_batch_create(ctx, descs, count, d_targets, d_row_ptr, &b);
+    if (rc) return rc;
+    rc = ppa_batch_launch(b, stream);
+    if (!rc) rc = ppa_batch_sync(b, stream);
+    ppa_batch_destroy(b);
+    return rc;
+}
+
+int ppa_autotune(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, int32_t *d_targets,
+                 int64_t *d_row_ptr, void *stream, ppa_tuning *best_out)
+{
+    if (!ctx || !descs || !count || !d_targets) return PPA_ERR_INVALID;
+    // workload signature: n bucket, d, p bucket (single graph) or batch shape
+    std::string sig;
+    if (count == 1) {
+        const ppa_graph_desc &g = descs[0];
+        const int nb = (int)std::floor(std::log2((double)g.n));
+        const int pb = (int)std::lround((double)g.p_threshold / 4294967296.0 * 10.0);
+        sig = "g:" + std::to_string(nb) + ":" + std::to_string(g.d) + ":" + std::to_string(pb);
+    } else {
+        double work = 0, dsum = 0;
+        for (uint64_t i = 0; i < count; ++i) {
+            work += (double)descs[i].n;
+            dsum += descs[i].d;
+        }
+        sig = "b:" + std::to_string((int)std::floor(std::log2((double)count))) + ":" +
+              std::to_string((int)std::floor(std::log2(work))) + ":" + std::to_string((int)(dsum / count));
+    }
+    const ppa_tuning base = ctx->tuning;
+    auto it = ctx->tune_cache.find(sig);
+    if (it != ctx->tune_cache.end()) {
+        ppa_tuning t = it->second;
+        t.watchdog_ms = base.watchdog_ms;
+        ctx->tuning = t;
+        if (best_out) *best_out = t;
+        return PPA_OK;
+    }
+    auto measure = [&](const ppa_tuning &t, double *ms) -> int {
+        ctx->tuning = t;
+        int rc;
+        if (count == 1) {
+            rc = ppa_generate(ctx, &descs[0], d_targets, d_row_ptr, stream);
+        } else {
+            ppa_batch *b = nullptr;
+            rc = ppa_batch_create(ctx, descs, count, d_targets, d_row_ptr, &b);
+            if (!rc) {
+                rc = ppa_batch_launch(b, stream);
+                if (!rc) rc = ppa_batch_sync(b, stream);
+                ppa_batch_destroy(b);
+            }
+        }
+        *ms = ctx->last.total_ms;
+        return rc;
+    };
+    // coordinate descent over the launch shape, two passes
+    ppa_tuning best = base;
+    double best_ms = 0;
+    int rc = measure(best, &best_ms);
+    if (rc) {
+        ctx->tuning = base;
+        return rc;
+    }
+    const uint32_t threads[] = {128, 256};
+    const uint32_t chunks[] = {64, 128, 256, 512, 1024};
+    const uint32_t per_cu[] = {0, 4, 6};
+    const uint32_t early[] = {1u << 18, 1u << 20, 1u << 22, 1u << 24};
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int axis = 0; axis < 4; ++axis) {
+            const uint32_t *vals = axis == 0 ? threads : axis == 1 ? chunks : axis == 2 ? per_cu : early;
+            const int nv = axis == 0 ? 2 : axis == 1 ? 5 : axis == 2 ? 3 : (count == 1 ? 4 : 0);
+            for (int v = 0; v < nv; ++v) {
+                ppa_tuning t = best;
+                uint32_t *field = axis == 0 ? &t.block_threads
+                                : axis == 1 ? &t.chunk_nodes
+                                : axis == 2 ? &t.blocks_per_cu
+                                            : &t.early_nodes;
+                if (*field == vals[v]) continue;
+                *field = vals[v];
+                double ms = 0;
+                if ((rc = measure(t, &ms)) != PPA_OK) {
+                    ctx->tuning = base;
+                    return rc;
+                }
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = t;
+                }
+            }
+        }
+    }
+    ctx->tuning = best;
+    ctx->tune_cache[sig] = best;
+    if (best_out) *best_out = best;
+    return PPA_OK;
+}
+
+}  // extern "C"

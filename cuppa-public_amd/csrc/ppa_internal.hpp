@@ -1,0 +1,91 @@
+// ppa_internal.hpp -- context state shared by the C ABI implementation files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ppa.h"
+#include "ppa_kernels.hpp"
+
+struct ppa_ctx {
+    int device = 0;
+    int num_cus = 0;
+    int wall_clock_khz = 0;
+    ppa_tuning tuning{};
+    ppa::DeviceScratch *scratch = nullptr;  // device
+    uint32_t *degree_scratch = nullptr;     // device, grown on demand
+    uint64_t degree_scratch_len = 0;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    ppa_stats last{};
+    bool pending = false;        // a launch awaits ppa_ctx_sync / ppa_batch_sync
+    uint64_t expect_nodes = 0;   // generated nodes the pending launch must finish
+    std::map<std::string, ppa_tuning> tune_cache;
+};
+
+namespace ppa {
+
+constexpr uint32_t kDefaultBlock = 256;
+constexpr uint32_t kDefaultChunk = 256;
+constexpr uint32_t kDefaultWatchdogMs = 60000;
+constexpr uint32_t kDefaultEarlyNodes = 1u << 22;
+constexpr uint32_t kEarlyChunk = 64;
+
+class DeviceGuard {
+public:
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev_) != hipSuccess) prev_ = -1;
+        ok_ = hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceGuard()
+    {
+        if (prev_ >= 0) (void)hipSetDevice(prev_);
+    }
+    bool ok() const { return ok_; }
+
+private:
+    int prev_ = -1;
+    bool ok_ = false;
+};
+
+#define PPA_HIP(call)                                    \
+    do {                                                 \
+        if ((call) != hipSuccess) return PPA_ERR_DEVICE; \
+    } while (0)
+
+struct LaunchShape {
+    uint32_t blocks = 0, threads = 0, chunk = 0;
+    uint64_t early_nodes = 0;
+    uint64_t watchdog_ticks = 0;
+};
+
+// Grid, workgroup and claim sizes for `work` generated nodes with out-degree
+// d (d = 0: the batch kernel).
+int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *out);
+
+// Host image of a batch's device tables, packed as
+// BatchGraph[count] | work_base[count+1] | slot_base[count+1] | row_base[count+1].
+struct BatchTables {
+    std::vector<char> bytes;
+    uint64_t count = 0;
+    uint64_t work = 0;         // generated nodes
+    uint64_t total_slots = 0;  // sum of n*d
+    uint64_t total_rows = 0;   // sum of n+1 over graphs with a row_ptr
+    bool any_row_ptr = false;
+};
+size_t batch_tables_bytes(uint64_t count);
+// graphs[i].G / row_ptr / targets_offset must be set; the rest is derived here.
+void build_batch_tables(const std::vector<BatchGraph> &graphs, uint64_t chunk, BatchTables *out);
+// reset + fill + generate of a batch whose tables are already at d_tables.
+hipError_t enqueue_batch(void *d_tables, const BatchTables &t, const LaunchShape &shape, DeviceScratch *scratch,
+                         hipStream_t stream);
+// A descriptor -> the kernel's graph record (G, row_ptr and offsets set by the caller).
+BatchGraph graph_record(const ppa_graph_desc &desc);
+// Waits for the stream and turns the device scratch into ctx->last.
+int finish_launch(ppa_ctx *ctx, hipStream_t stream);
+bool tuning_valid(const ppa_tuning &t);
+
+}  // namespace ppa

@@ -2,7 +2,7 @@
 //
 // Reference kernels replaced here (src/PA/cuPPA/):
 //   initial_graph_rrp (cuPPA_RRP.cu:51-76) and the per-stage
-//   initialize_vertex_structures_rrp (cuPPA_RRP.cu:78-102)  -> fill_kernel
+//   initialize_vertex_structures_rrp (cuPPA_RRP.cu:78-102)  -> fill_kernel, fill_batch_kernel
 //   generate_graph_rrp_by_batch (cuPPA_RRP_1.cu:28-209),
 //   re_execute_copy_model_rrp (cuPPA_RRP.cu:9-49), the cuBSTVertex duplicate
 //   check (cuBSTVertex.cuh:153-319) and the host stage loop
@@ -16,10 +16,10 @@
 //     consecutive nodes from one global counter, in ascending order, and hand
 //     them to their idle lanes in ascending order. A slot only ever waits on a
 //     slot of a smaller node, and the smallest unfinished node is always owned
-//     by a lane whose own dependencies are all final, so the grid always makes
-//     progress, whatever the residency of workgroups.
+//     by a lane whose own dependencies are all final and published, so the
+//     grid always makes progress, whatever the residency of workgroups.
 //   * Slots are write-once: the fill kernel sets every generated slot to -1
-//     before the launch, and the owner later stores the final value with an
+//     before the launch and the owner later stores the final value with an
 //     agent-scope store. A plain (L2-cached) read therefore returns either -1
 //     or the final value; only a -1 needs a coherent (agent-scope) re-read.
 //   * On claiming a node, a lane draws attempt 0 of all d slots at once and
@@ -28,55 +28,64 @@
 //     rejected attempt (duplicate) or an empty source slot costs another
 //     round trip. The reference instead parks placeholders -(v*M+k) in a
 //     per-thread LDS queue and spins (cuPPA_RRP_1.cu:97-185).
-//   * The per-node duplicate set is the node's own d registers: d <= 16 means
-//     an unrolled compare against at most 15 values, no tree walk.
+//   * Finished rows are published per loop iteration through LDS: the rows
+//     finished by a wavefront within 64 consecutive nodes are transposed so
+//     that each store instruction writes whole, contiguous 32-byte sectors
+//     (agent-scope buffer stores). Per-lane stores of a 32-byte row would cost
+//     a partial-sector write per 4 or 8 bytes.
+//   * Below `early_nodes`, where almost every copy source is still being
+//     generated, each slot is published as soon as it is accepted and claims
+//     are one node per lane, which shortens the dependency chains.
+#include "ppa_device.hpp"
 #include "ppa_kernels.hpp"
 #include "ppa_model.hpp"
 
 namespace ppa {
 namespace {
 
-constexpr int kWave = 64;
-constexpr uint32_t kMaxRetry = 1u << 24;
-constexpr int kErrRetry = -2;
-constexpr int kErrTimeout = -5;
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+constexpr int kBufferFlags = 0x00020000;  // gfx9 buffer descriptor word 3 (32-bit data format)
+constexpr int kSc1 = 16;                  // buffer-store cache policy: agent scope (sc1)
 
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-__device__ __forceinline__ uint64_t uniform64(uint64_t v)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(void *base, uint32_t bytes)
 {
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
+    return __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)bytes, kBufferFlags);
 }
 
-__device__ __forceinline__ uint32_t rank_below(uint64_t mask)
-{
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-__device__ __forceinline__ int32_t load_coherent(const int32_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void store_coherent(int32_t *p, int32_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
 {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, off, kWave);
+        const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), off, kWave);
+        const uint64_t o = ((uint64_t)hi << 32) | lo;
+        v = o < v ? o : v;
+    }
     return v;
 }
 
-__global__ void reset_scratch_kernel(DeviceScratch *s, uint64_t first_node)
+__device__ __forceinline__ uint64_t wave_or_u64(uint64_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, off, kWave);
+        const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), off, kWave);
+        v |= ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t lane)
+{
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, (int)lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__global__ void reset_scratch_kernel(DeviceScratch *s, uint64_t first)
 {
     if (threadIdx.x == 0) {
-        s->claim = first_node;
+        s->claim = first;
         s->status = 0;
         s->max_degree = 0;
     }
@@ -113,60 +122,146 @@ __global__ void fill_kernel(FillArgs a)
         for (uint64_t t = tid; t <= a.n; t += stride) a.row_ptr[t] = (int64_t)(t * a.d);
 }
 
-template <int D>
+// Largest i in [0, count) with base[i] <= x (base ascending, base[0] = 0).
+__device__ __forceinline__ uint64_t find_segment(const uint64_t *base, uint64_t count, uint64_t x)
+{
+    uint64_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (base[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+constexpr uint32_t kFillPerThread = 8;
+
+// The fill of every graph of a batch over one flat index space of slots and
+// one of row offsets; a block locates its first graph once.
+__global__ __launch_bounds__(256) void fill_batch_kernel(BatchFillArgs a, uint64_t total_slots,
+                                                          uint64_t total_rows)
+{
+    const uint64_t per_block = (uint64_t)blockDim.x * kFillPerThread;
+    for (uint64_t base = blockIdx.x * per_block; base < total_slots; base += gridDim.x * per_block) {
+        uint64_t g = find_segment(a.slot_base, a.count, base);
+        for (uint32_t j = 0; j < kFillPerThread; ++j) {
+            const uint64_t s = base + (uint64_t)j * blockDim.x + threadIdx.x;
+            if (s >= total_slots) break;
+            while (s >= a.slot_base[g + 1]) ++g;
+            const BatchGraph &gr = a.graphs[g];
+            const uint64_t local = s - a.slot_base[g];
+            const uint64_t clique = (uint64_t)(gr.d + 1) * gr.d;
+            gr.G[local] = local < clique ? clique_target(local / gr.d, (uint32_t)(local % gr.d)) : kEmpty;
+        }
+    }
+    for (uint64_t base = blockIdx.x * per_block; base < total_rows; base += gridDim.x * per_block) {
+        uint64_t g = find_segment(a.row_base, a.count, base);
+        for (uint32_t j = 0; j < kFillPerThread; ++j) {
+            const uint64_t s = base + (uint64_t)j * blockDim.x + threadIdx.x;
+            if (s >= total_rows) break;
+            while (s >= a.row_base[g + 1]) ++g;
+            const BatchGraph &gr = a.graphs[g];
+            if (gr.row_ptr) {
+                const uint64_t t = s - a.row_base[g];
+                gr.row_ptr[t] = (int64_t)(gr.targets_offset + t * gr.d);
+            }
+        }
+    }
+}
+
+// The generator. DMAX is the out-degree (single graph, BATCH = false) or
+// its upper bound (BATCH = true: every lane carries its graph's parameters).
+template <int DMAX, bool BATCH>
 __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
 {
+    __shared__ __attribute__((aligned(16))) int32_t stage_lds[4][64 * DMAX];
     const uint32_t lane = lane_id();
-    int32_t *const G = a.G;
-    const uint64_t n = a.n;
-    const uint64_t T = a.T;
+    int32_t *const stage = stage_lds[threadIdx.x >> 6];
     const uint64_t start_clock = wall_clock64();
 
-    // Wavefront-uniform claim window [wcur, wend).
-    uint64_t wcur = 0, wend = 0;
+    // Wavefront-uniform claim window [wcur, wend) of node (or work) indices,
+    // and for batches the window's graph.
+    uint64_t wcur = 0, wend = 0, wlast = 0;
     bool wdone = false;
+    int32_t *wG = a.G;
+    uint64_t wT = a.T, wtoff = 0;
+    uint32_t wk0 = a.key0, wk1 = a.key1, wd = DMAX, wgid = 0;
+    double winv = 0.0;
 
     // Lane state.
     uint64_t t = 0;
-    bool active = false, fresh = false, need_draw = false, have_src = false;
+    bool active = false, fresh = false, need_draw = false, have_src = false, early = false;
     uint32_t k = 0, r = 0;
     uint64_t src = 0;
-    int32_t val[D];
+    int32_t *G = a.G;
+    uint64_t T = a.T;
+    uint32_t k0 = a.key0, k1 = a.key1, d = DMAX, gid = 0;
+    double inv_d = 0.0;
+    int32_t val[DMAX];
 #pragma unroll
-    for (int i = 0; i < D; ++i) val[i] = kEmpty;
+    for (int i = 0; i < DMAX; ++i) val[i] = kEmpty;
 
-    uint32_t c_nodes = 0, c_draws = 0, c_retries = 0, c_pmiss = 0, c_polls = 0, c_stalls = 0,
-             c_claims = 0, c_sleeps = 0;
+    uint32_t c_nodes = 0, c_slots = 0, c_draws = 0, c_retries = 0, c_pmiss = 0, c_polls = 0, c_stalls = 0,
+             c_claims = 0, c_sleeps = 0, c_staged = 0;
     uint32_t iter = 0, backoff = 1;
     int fail = 0;
 
     while (true) {
-        // ---- claim nodes for idle lanes (wavefront-uniform control flow) ----
+        // ---- claim work for idle lanes (wavefront-uniform control flow) ----
         bool progressed = false;
         const uint64_t idle_mask = __ballot(!active);
         if (!wdone && idle_mask != 0) {
             if (wcur >= wend) {
+                const uint32_t size = (!BATCH && wlast < a.early_nodes) ? a.early_chunk : a.chunk;
                 uint64_t base = 0;
                 if (lane == 0) {
-                    base = atomicAdd(&a.scratch->claim, (unsigned long long)a.chunk);
+                    base = atomicAdd(&a.scratch->claim, (unsigned long long)size);
                     ++c_claims;
                 }
                 base = uniform64(base);
-                if (base >= n) {
+                wlast = base;
+                if (BATCH) {
+                    if (base >= a.work_base[a.count]) {
+                        wdone = true;
+                    } else {
+                        wgid = (uint32_t)find_segment(a.work_base, a.count, base);
+                        const BatchGraph &gr = a.graphs[wgid];
+                        const uint64_t gend = a.work_base[wgid] + (gr.n - gr.d - 1);
+                        wcur = base;
+                        wend = base + size < gend ? base + size : gend;
+                        wG = gr.G;
+                        wT = gr.T;
+                        wk0 = gr.key0;
+                        wk1 = gr.key1;
+                        wd = gr.d;
+                        winv = gr.inv_d;
+                        wtoff = (uint64_t)gr.d + 1 - a.work_base[wgid];
+                    }
+                } else if (base >= a.n) {
                     wdone = true;
                 } else {
                     wcur = base;
-                    wend = base + a.chunk < n ? base + a.chunk : n;
+                    wend = base + size < a.n ? base + size : a.n;
                 }
             }
             if (!wdone) {
                 const uint64_t avail = wend - wcur;
                 const uint32_t rank = rank_below(idle_mask);
                 if (!active && rank < avail) {
-                    t = wcur + rank;
+                    t = wcur + rank + wtoff;
                     active = true;
                     fresh = true;
                     progressed = true;
+                    if (BATCH) {
+                        G = wG;
+                        T = wT;
+                        k0 = wk0;
+                        k1 = wk1;
+                        d = wd;
+                        inv_d = winv;
+                        gid = wgid;
+                    }
+                    early = t < a.early_nodes;
                 }
                 const uint64_t idle = (uint64_t)__popcll(idle_mask);
                 wcur += idle < avail ? idle : avail;
@@ -174,18 +269,22 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
         }
         if (wdone && __ballot(active) == 0) break;
 
-        const uint64_t td = t * (uint64_t)D;
+        const uint32_t dd = BATCH ? d : (uint32_t)DMAX;
+        const uint64_t td = t * (uint64_t)dd;
+        auto node_of = [&](uint64_t pair) -> int32_t {
+            return (int32_t)(BATCH ? div_small(pair, dd, inv_d) : pair / (uint64_t)DMAX);
+        };
         if (fresh) {
             // Attempt 0 of every slot at once; copy reads go out together.
 #pragma unroll
-            for (int kk = 0; kk < D; ++kk) {
-                const Attempt at = attempt(a.key0, a.key1, T, t, (uint32_t)kk, 0u, td);
-                if (at.direct)
-                    val[kk] = (int32_t)(at.pair / (uint64_t)D);
-                else
-                    val[kk] = G[at.pair];
+            for (int kk = 0; kk < DMAX; ++kk) {
+                if (!BATCH || (uint32_t)kk < dd) {
+                    const Attempt at = attempt(k0, k1, T, t, (uint32_t)kk, 0u, td);
+                    if (at.direct) val[kk] = node_of(at.pair);
+                    else val[kk] = __builtin_nontemporal_load(G + at.pair);
+                }
             }
-            c_draws += D;
+            c_draws += dd;
             k = 0;
             r = 0;
             need_draw = false;
@@ -196,13 +295,12 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
             int32_t got = kEmpty;
             bool resolved = false;
             if (need_draw) {
-                const Attempt at = attempt(a.key0, a.key1, T, t, k, r, td);
+                const Attempt at = attempt(k0, k1, T, t, k, r, td);
                 ++c_draws;
                 need_draw = false;
                 if (at.direct) {
-                    got = (int32_t)(at.pair / (uint64_t)D);
+                    got = node_of(at.pair);
                     resolved = true;
-                    progressed = true;
                 } else {
                     src = at.pair;
                     have_src = true;
@@ -213,25 +311,26 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
                 ++c_polls;
                 if (got != kEmpty) {
                     resolved = true;
-                    progressed = true;
                     have_src = false;
                 } else {
                     ++c_stalls;
                 }
             }
             if (resolved) {
+                progressed = true;
 #pragma unroll
-                for (int kk = 0; kk < D; ++kk)
+                for (int kk = 0; kk < DMAX; ++kk)
                     if ((uint32_t)kk == k) val[kk] = got;
             }
         }
 
         // ---- accept slots in order: duplicate check against filled slots ----
+        bool finished = false;
         if (active && !need_draw && !have_src) {
             bool stop = false;
 #pragma unroll
-            for (int kk = 0; kk < D; ++kk) {
-                if (!stop && (uint32_t)kk == k) {
+            for (int kk = 0; kk < DMAX; ++kk) {
+                if (!stop && (uint32_t)kk == k && (!BATCH || (uint32_t)kk < dd)) {
                     const int32_t c = val[kk];
                     if (c == kEmpty) {
                         // attempt 0's copy source was still empty: re-derive it
@@ -249,6 +348,7 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
                             stop = true;
                             if (r >= kMaxRetry) fail = kErrRetry;
                         } else {
+                            if (early) store_coherent(G + td + kk, c);
                             ++k;
                             r = 0;
                             progressed = true;
@@ -256,13 +356,67 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
                     }
                 }
             }
-            if (k == (uint32_t)D) {
-                int32_t *row = G + td;
-#pragma unroll
-                for (int kk = 0; kk < D; ++kk) store_coherent(row + kk, val[kk]);
+            if (k == dd) {
                 active = false;
+                finished = true;
                 ++c_nodes;
+                c_slots += dd;
             }
+        }
+
+        // ---- publish finished rows: coalesced through LDS (wavefront-uniform) ----
+        const bool want = finished && !early;
+        if (__ballot(want) != 0) {
+            const uint64_t key = want ? (((uint64_t)gid << 32) | t) : ~0ull;
+            const uint64_t kmin = wave_min_u64(key);
+            const bool in_range = want && key - kmin < 64;
+            // bit r set <=> staged row r (node t0 + r) is finished this iteration;
+            // indexed by row, not by lane
+            const uint64_t rmask = wave_or_u64(in_range ? 1ull << (key - kmin) : 0ull);
+            const uint32_t lead = (uint32_t)__builtin_ctzll(__ballot(want && key == kmin));
+            const uint32_t Ld = BATCH ? (uint32_t)__builtin_amdgcn_readlane((int)dd, (int)lead) : (uint32_t)DMAX;
+            int32_t *const LG = BATCH ? (int32_t *)readlane64((uint64_t)G, lead) : a.G;
+            const uint64_t t0 = kmin & 0xffffffffull;
+            if (in_range) {
+                const uint32_t row = (uint32_t)(key - kmin);
+#pragma unroll
+                for (int kk = 0; kk < DMAX; ++kk)
+                    if (!BATCH || (uint32_t)kk < dd) stage[row * Ld + kk] = val[kk];
+                ++c_staged;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const __amdgpu_buffer_rsrc_t out = rsrc_of(LG + t0 * Ld, 64u * Ld * 4u);
+            if (!BATCH && DMAX % 4 == 0) {
+                // 16-byte pieces: piece p covers ints [4p, 4p + 4) of the staged rows
+#pragma unroll
+                for (int i = 0; i < DMAX / 4; ++i) {
+                    const uint32_t p = (uint32_t)i * 64u + lane;
+                    if ((rmask >> ((p * 4u) / (uint32_t)DMAX)) & 1ull) {
+                        const v4u v = *reinterpret_cast<const v4u *>(&stage[p * 4u]);
+                        __builtin_amdgcn_raw_buffer_store_b128(v, out, (int)(p * 16u), 0, kSc1);
+                    }
+                }
+            } else {
+                // 4-byte pieces: int e of the staged rows belongs to row e / Ld
+                const uint32_t magic = (65536u + Ld - 1u) / Ld;  // exact for e < 1024, Ld <= 16
+#pragma unroll
+                for (int i = 0; i < DMAX; ++i) {
+                    if ((uint32_t)i < Ld) {
+                        const uint32_t e = (uint32_t)i * 64u + lane;
+                        const uint32_t row = BATCH ? (e * magic) >> 16 : e / (uint32_t)DMAX;
+                        if ((rmask >> row) & 1ull)
+                            __builtin_amdgcn_raw_buffer_store_b32((uint32_t)stage[e], out, (int)(e * 4u), 0, kSc1);
+                    }
+                }
+            }
+            if (want && !in_range) {
+#pragma unroll
+                for (int kk = 0; kk < DMAX; ++kk)
+                    if (!BATCH || (uint32_t)kk < dd) store_coherent(G + td + kk, val[kk]);
+            }
+            __builtin_amdgcn_wave_barrier();
         }
 
         // ---- failure, watchdog and back-off (wavefront-uniform) ----
@@ -284,7 +438,7 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
         }
         if (__ballot(progressed) == 0) {
             // Every lane of this wavefront is waiting on another node.
-            ++c_sleeps;
+            if (lane == 0) ++c_sleeps;
             for (uint32_t s = 0; s < backoff; ++s) __builtin_amdgcn_s_sleep(4);
             backoff = backoff < 32 ? backoff * 2 : 32;
         } else {
@@ -293,20 +447,16 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
     }
 
     // ---- statistics: one atomic per counter per wavefront ----
-    const uint32_t sums[8] = {wave_sum(c_nodes), wave_sum(c_draws), wave_sum(c_retries),
-                              wave_sum(c_pmiss), wave_sum(c_polls), wave_sum(c_stalls),
-                              wave_sum(c_claims), wave_sum(c_sleeps)};
+    const uint32_t sums[10] = {wave_sum(c_nodes),  wave_sum(c_slots),  wave_sum(c_draws),  wave_sum(c_retries),
+                               wave_sum(c_pmiss),  wave_sum(c_polls),  wave_sum(c_stalls), wave_sum(c_claims),
+                               wave_sum(c_sleeps), wave_sum(c_staged)};
     if (lane == 0) {
         unsigned long long *st = a.scratch->stats;
-        atomicAdd(&st[kStatNodes], (unsigned long long)sums[0]);
-        atomicAdd(&st[kStatSlots], (unsigned long long)sums[0] * D);
-        atomicAdd(&st[kStatDraws], (unsigned long long)sums[1]);
-        atomicAdd(&st[kStatRetries], (unsigned long long)sums[2]);
-        atomicAdd(&st[kStatPrefetchMisses], (unsigned long long)sums[3]);
-        atomicAdd(&st[kStatPolls], (unsigned long long)sums[4]);
-        atomicAdd(&st[kStatStalls], (unsigned long long)sums[5]);
-        atomicAdd(&st[kStatClaims], (unsigned long long)sums[6]);
-        atomicAdd(&st[kStatSleeps], (unsigned long long)sums[7]);
+        const int idx[10] = {kStatNodes,  kStatSlots,  kStatDraws,  kStatRetries, kStatPrefetchMisses,
+                             kStatPolls,  kStatStalls, kStatClaims, kStatSleeps,  kStatStagedRows};
+#pragma unroll
+        for (int i = 0; i < 10; ++i)
+            if (sums[i]) atomicAdd(&st[idx[i]], (unsigned long long)sums[i]);
     }
 }
 
@@ -378,24 +528,24 @@ uint32_t grid_for(uint64_t work, uint32_t block, uint32_t cap)
     return (uint32_t)g;
 }
 
-template <int D>
+template <int D, bool BATCH>
 hipError_t launch_generate_d(const GenArgs &a, uint32_t blocks, uint32_t threads, hipStream_t stream)
 {
-    hipLaunchKernelGGL(generate_kernel<D>, dim3(blocks), dim3(threads), 0, stream, a);
+    hipLaunchKernelGGL((generate_kernel<D, BATCH>), dim3(blocks), dim3(threads), 0, stream, a);
     return hipGetLastError();
 }
 
-template <int D>
+template <int D, bool BATCH>
 hipError_t occupancy_d(uint32_t threads, int *out)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, generate_kernel<D>, (int)threads, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, generate_kernel<D, BATCH>, (int)threads, 0);
 }
 
 }  // namespace
 
-hipError_t launch_reset_scratch(DeviceScratch *s, uint64_t first_node, hipStream_t stream)
+hipError_t launch_reset_scratch(DeviceScratch *s, uint64_t first, hipStream_t stream)
 {
-    hipLaunchKernelGGL(reset_scratch_kernel, dim3(1), dim3(64), 0, stream, s, first_node);
+    hipLaunchKernelGGL(reset_scratch_kernel, dim3(1), dim3(64), 0, stream, s, first);
     return hipGetLastError();
 }
 
@@ -403,6 +553,15 @@ hipError_t launch_fill(const FillArgs &a, hipStream_t stream)
 {
     const uint64_t work = (a.n * a.d) / 4 + 1;
     hipLaunchKernelGGL(fill_kernel, dim3(grid_for(work, 256, 8192)), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_batch(const BatchFillArgs &a, uint64_t total_slots, uint64_t total_rows,
+                             hipStream_t stream)
+{
+    const uint64_t work = (total_slots > total_rows ? total_slots : total_rows) / kFillPerThread + 1;
+    hipLaunchKernelGGL(fill_batch_kernel, dim3(grid_for(work, 256, 16384)), dim3(256), 0, stream, a,
+                       total_slots, total_rows);
     return hipGetLastError();
 }
 
@@ -414,18 +573,24 @@ hipError_t launch_generate(uint32_t d, const GenArgs &a, uint32_t blocks, uint32
 {
     switch (d) {
 #define PPA_CASE(DD) \
-    case DD: return launch_generate_d<DD>(a, blocks, threads, stream);
+    case DD: return launch_generate_d<DD, false>(a, blocks, threads, stream);
         PPA_FOR_EACH_D(PPA_CASE)
 #undef PPA_CASE
     default: return hipErrorInvalidValue;
     }
 }
 
+hipError_t launch_generate_batch(const GenArgs &a, uint32_t blocks, uint32_t threads, hipStream_t stream)
+{
+    return launch_generate_d<kBatchDMax, true>(a, blocks, threads, stream);
+}
+
 hipError_t generate_occupancy(uint32_t d, uint32_t threads, int *out)
 {
     switch (d) {
+    case 0: return occupancy_d<kBatchDMax, true>(threads, out);
 #define PPA_CASE(DD) \
-    case DD: return occupancy_d<DD>(threads, out);
+    case DD: return occupancy_d<DD, false>(threads, out);
         PPA_FOR_EACH_D(PPA_CASE)
 #undef PPA_CASE
     default: return hipErrorInvalidValue;

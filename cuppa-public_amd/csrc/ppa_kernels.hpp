@@ -6,7 +6,6 @@
 
 namespace ppa {
 
-// Device scratch of one context (one per call in flight on that context).
 enum StatIndex : int {
     kStatNodes = 0,
     kStatSlots,
@@ -17,24 +16,48 @@ enum StatIndex : int {
     kStatStalls,
     kStatClaims,
     kStatSleeps,
+    kStatStagedRows,
     kStatCount
 };
 
+// Device scratch of one context (one call in flight per context).
 struct DeviceScratch {
-    unsigned long long claim;  // next unclaimed node
+    unsigned long long claim;  // next unclaimed node (single graph) or work index (batch)
     int status;                // 0 = running/ok, else a negative PPA_ERR_* code
     int pad;
     unsigned long long stats[kStatCount];
     unsigned long long max_degree;
 };
 
-struct GenArgs {
-    int32_t *G;           // targets, row-major [n][d]
+// One graph of a batch, as the kernels read it (64 bytes).
+struct BatchGraph {
+    int32_t *G;               // this graph's targets, row-major [n][d]
+    int64_t *row_ptr;         // this graph's row offsets [n+1], nullable
     uint64_t n;
-    uint64_t T;           // coin threshold
-    uint32_t key0, key1;  // Philox key
-    uint32_t chunk;       // nodes claimed per wavefront claim
+    uint64_t T;               // coin threshold
+    uint64_t targets_offset;  // added to every row_ptr entry
+    uint32_t key0, key1;      // Philox key
+    uint32_t d;
     uint32_t pad;
+    double inv_d;             // 1.0 / d
+};
+static_assert(sizeof(BatchGraph) == 64, "BatchGraph layout");
+
+struct GenArgs {
+    // single graph
+    int32_t *G;               // targets, row-major [n][d]
+    uint64_t n;
+    uint64_t T;
+    uint32_t key0, key1;
+    // batch: graphs[i] owns work indices [work_base[i], work_base[i] + n_i - d_i - 1);
+    // work_base is a chunk-aligned exclusive prefix with count + 1 entries
+    const BatchGraph *graphs;
+    const uint64_t *work_base;
+    uint64_t count;
+    // common
+    uint32_t chunk;           // work claimed per wavefront claim (multiple of 64)
+    uint32_t early_chunk;     // claim size below early_nodes (single graph)
+    uint64_t early_nodes;     // below this node index every slot is published as accepted
     uint64_t watchdog_ticks;  // wall-clock ticks a wavefront may run
     DeviceScratch *scratch;
 };
@@ -47,13 +70,24 @@ struct FillArgs {
     uint32_t pad;
 };
 
-// Kernels (defined in ppa_kernels.hip).
-hipError_t launch_reset_scratch(DeviceScratch *s, uint64_t first_node, hipStream_t stream);
+struct BatchFillArgs {
+    const BatchGraph *graphs;
+    const uint64_t *slot_base;  // exclusive prefix of n_i * d_i, count + 1 entries
+    const uint64_t *row_base;   // exclusive prefix of n_i + 1, count + 1 entries
+    uint64_t count;
+};
+
+hipError_t launch_reset_scratch(DeviceScratch *s, uint64_t first, hipStream_t stream);
 hipError_t launch_fill(const FillArgs &a, hipStream_t stream);
+hipError_t launch_fill_batch(const BatchFillArgs &a, uint64_t total_slots, uint64_t total_rows,
+                             hipStream_t stream);
 hipError_t launch_generate(uint32_t d, const GenArgs &a, uint32_t blocks, uint32_t block_threads,
                            hipStream_t stream);
+hipError_t launch_generate_batch(const GenArgs &a, uint32_t blocks, uint32_t block_threads, hipStream_t stream);
 hipError_t generate_occupancy(uint32_t d, uint32_t block_threads, int *blocks_per_cu);
 hipError_t launch_degree(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, uint64_t *hist,
                          uint64_t hist_len, DeviceScratch *s, hipStream_t stream);
+
+constexpr uint32_t kBatchDMax = 16;  // d of 0 selects the batch kernel in generate_occupancy
 
 }  // namespace ppa

@@ -47,20 +47,44 @@ class GraphDesc(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("block_threads", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
                 ("chunk_nodes", ctypes.c_uint32), ("max_blocks", ctypes.c_uint32),
-                ("watchdog_ms", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+                ("watchdog_ms", ctypes.c_uint32), ("early_nodes", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 2)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("nodes", ctypes.c_uint64), ("slots", ctypes.c_uint64), ("draws", ctypes.c_uint64),
                 ("retries", ctypes.c_uint64), ("prefetch_misses", ctypes.c_uint64),
                 ("polls", ctypes.c_uint64), ("stalls", ctypes.c_uint64), ("claims", ctypes.c_uint64),
-                ("sleeps", ctypes.c_uint64), ("grid_blocks", ctypes.c_uint64),
+                ("sleeps", ctypes.c_uint64), ("staged_rows", ctypes.c_uint64), ("grid_blocks", ctypes.c_uint64),
                 ("block_threads", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_uint64), ("streams", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
+                ("batch_nodes", ctypes.c_uint64), ("solo_nodes", ctypes.c_uint64), ("plain", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("n", ctypes.c_uint64), ("d", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("p_threshold", ctypes.c_uint64), ("seed", ctypes.c_uint64)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32), ("targets", ctypes.c_void_p),
+                ("row_ptr", ctypes.c_void_p), ("n", ctypes.c_uint64), ("d", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("submit_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
 
 _lib = None
 
@@ -80,6 +104,25 @@ _SIGNATURES = [
     ("ppa_ctx_sync", ctypes.c_int, [_VP, _VP]),
     ("ppa_degree_histogram", ctypes.c_int,
      [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP, ctypes.c_uint64, _VP, _VP]),
+    ("ppa_batch_create", ctypes.c_int,
+     [_VP, ctypes.POINTER(GraphDesc), ctypes.c_uint64, _VP, _VP, ctypes.POINTER(_VP)]),
+    ("ppa_batch_launch", ctypes.c_int, [_VP, _VP]),
+    ("ppa_batch_sync", ctypes.c_int, [_VP, _VP]),
+    ("ppa_batch_destroy", ctypes.c_int, [_VP]),
+    ("ppa_generate_batch", ctypes.c_int, [_VP, ctypes.POINTER(GraphDesc), ctypes.c_uint64, _VP, _VP, _VP]),
+    ("ppa_autotune", ctypes.c_int,
+     [_VP, ctypes.POINTER(GraphDesc), ctypes.c_uint64, _VP, _VP, _VP, ctypes.POINTER(Tuning)]),
+    ("ppa_server_create", ctypes.c_int, [_VP, ctypes.POINTER(ServerConfig), ctypes.POINTER(_VP)]),
+    ("ppa_server_destroy", ctypes.c_int, [_VP]),
+    ("ppa_server_submit", ctypes.c_int, [_VP, ctypes.POINTER(Request)]),
+    ("ppa_server_step", ctypes.c_int, [_VP]),
+    ("ppa_server_poll", ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.POINTER(Result)]),
+    ("ppa_server_release", ctypes.c_int, [_VP, ctypes.c_uint64]),
+    ("ppa_server_cancel", ctypes.c_int, [_VP, ctypes.c_uint64]),
+    ("ppa_server_usage", ctypes.c_int, [_VP] + [ctypes.POINTER(ctypes.c_uint64)] * 4),
+    ("ppa_server_replay", ctypes.c_int, [_VP, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double),
+                                         ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double)]),
 ]
 
 
@@ -172,9 +215,15 @@ class Context:
     def handle(self):
         return self._h
 
-    def set_tuning(self, block_threads=0, blocks_per_cu=0, chunk_nodes=0, max_blocks=0, watchdog_ms=0):
-        t = Tuning(block_threads, blocks_per_cu, chunk_nodes, max_blocks, watchdog_ms)
+    def set_tuning(self, block_threads=0, blocks_per_cu=0, chunk_nodes=0, max_blocks=0, watchdog_ms=0,
+                   early_nodes=0):
+        t = Tuning(block_threads, blocks_per_cu, chunk_nodes, max_blocks, watchdog_ms, early_nodes)
         _check(lib().ppa_ctx_set_tuning(self._h, ctypes.byref(t)), "ppa_ctx_set_tuning")
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().ppa_ctx_get_tuning(self._h, ctypes.byref(t)), "ppa_ctx_get_tuning")
+        return t.as_dict()
 
     def stats(self) -> dict:
         s = Stats()
@@ -224,6 +273,138 @@ class Context:
                                           ctypes.c_void_p(mx.data_ptr()), _stream_handle(stream)),
                "ppa_degree_histogram")
         return hist, int(mx.item()), deg
+
+
+def batch_layout(specs):
+    """specs: list of (n, d, p_threshold, seed). Returns (GraphDesc array,
+    total target slots, total row_ptr entries) with graphs laid out back to back."""
+    descs = (GraphDesc * len(specs))()
+    toff = roff = 0
+    for i, (n, d, T, seed) in enumerate(specs):
+        descs[i] = GraphDesc(n, d, 0, T, seed & 0xFFFFFFFFFFFFFFFF, toff, roff)
+        toff += n * d
+        roff += n + 1
+    return descs, toff, roff
+
+
+class Batch:
+    """A prepared batch (ppa_batch): descriptor table on the device and a
+    captured hipGraph of reset + fill + generate."""
+
+    def __init__(self, ctx: "Context", descs, targets, row_ptr=None):
+        self.ctx = ctx
+        self._h = ctypes.c_void_p()
+        self.descs = descs
+        self.targets, self.row_ptr = targets, row_ptr
+        _check(lib().ppa_batch_create(ctx.handle, descs, len(descs), ctypes.c_void_p(targets.data_ptr()),
+                                      ctypes.c_void_p(row_ptr.data_ptr()) if row_ptr is not None else None,
+                                      ctypes.byref(self._h)), "ppa_batch_create")
+
+    def launch(self, stream=None):
+        _check(lib().ppa_batch_launch(self._h, _stream_handle(stream)), "ppa_batch_launch")
+
+    def sync(self, stream=None):
+        _check(lib().ppa_batch_sync(self._h, _stream_handle(stream)), "ppa_batch_sync")
+
+    def run(self, stream=None):
+        self.launch(stream)
+        self.sync(stream)
+
+    def close(self):
+        if self._h:
+            lib().ppa_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(ctx: "Context", descs, targets, row_ptr=None, stream=None) -> dict:
+    t = Tuning()
+    _check(lib().ppa_autotune(ctx.handle, descs, len(descs), ctypes.c_void_p(targets.data_ptr()),
+                              ctypes.c_void_p(row_ptr.data_ptr()) if row_ptr is not None else None,
+                              _stream_handle(stream), ctypes.byref(t)), "ppa_autotune")
+    return t.as_dict()
+
+
+class Server:
+    """Continuous-batching request loop (ppa_server)."""
+
+    def __init__(self, ctx: "Context", budget_bytes: int, streams: int = 4, max_batch: int = 0,
+                 batch_nodes: int = 0, solo_nodes: int = 0, plain: bool = False):
+        self.ctx = ctx
+        self._h = ctypes.c_void_p()
+        cfg = ServerConfig(budget_bytes, streams, max_batch, batch_nodes, solo_nodes, 1 if plain else 0, 0)
+        _check(lib().ppa_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(self._h)), "ppa_server_create")
+
+    def submit(self, rid: int, n: int, d: int, p_thresh: int, seed: int) -> int:
+        return lib().ppa_server_submit(self._h, ctypes.byref(Request(rid, n, d, 0, p_thresh, seed)))
+
+    def step(self) -> int:
+        rc = lib().ppa_server_step(self._h)
+        if rc < 0:
+            raise PPAError(rc, "ppa_server_step")
+        return rc
+
+    def poll(self, rid: int) -> Result:
+        r = Result()
+        _check(lib().ppa_server_poll(self._h, rid, ctypes.byref(r)), "ppa_server_poll")
+        return r
+
+    def release(self, rid: int) -> int:
+        return lib().ppa_server_release(self._h, rid)
+
+    def cancel(self, rid: int) -> int:
+        return lib().ppa_server_cancel(self._h, rid)
+
+    def usage(self) -> dict:
+        v = [ctypes.c_uint64() for _ in range(4)]
+        _check(lib().ppa_server_usage(self._h, *[ctypes.byref(x) for x in v]), "ppa_server_usage")
+        return dict(zip(("used", "budget", "queued", "running"), (x.value for x in v)))
+
+    def replay(self, specs, arrival_ms):
+        """specs: list of (id, n, d, p_threshold, seed). Returns (latency_ms list, makespan_ms)."""
+        count = len(specs)
+        reqs = (Request * count)(*[Request(i, n, d, 0, T, s) for (i, n, d, T, s) in specs])
+        arr = (ctypes.c_double * count)(*arrival_ms)
+        lat = (ctypes.c_double * count)()
+        mk = ctypes.c_double()
+        _check(lib().ppa_server_replay(self._h, reqs, arr, count, lat, ctypes.byref(mk)), "ppa_server_replay")
+        return list(lat), mk.value
+
+    def close(self):
+        if self._h:
+            lib().ppa_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_hip = None
+
+
+def read_device(ptr: int, count: int, np_dtype):
+    """Copies `count` elements of device memory at `ptr` into a new numpy array
+    (hipMemcpy device-to-host)."""
+    import numpy as np
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipMemcpy.restype = ctypes.c_int
+    out = np.empty(count, dtype=np_dtype)
+    if count:
+        rc = _hip.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2)  # hipMemcpyDeviceToHost
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed: {rc}")
+    return out
 
 
 class CuPPA:

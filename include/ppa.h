@@ -77,7 +77,9 @@ typedef struct ppa_tuning {
     uint32_t chunk_nodes;    /* nodes a wavefront claims at once, multiple of 64 (0 = default) */
     uint32_t max_blocks;     /* cap on the persistent grid (0 = no cap) */
     uint32_t watchdog_ms;    /* kernel deadline (0 = default 60000) */
-    uint32_t reserved[3];
+    uint32_t early_nodes;    /* nodes below this index publish each slot as it is accepted and
+                                are claimed one per lane (0 = default 2^22, 1 = none) */
+    uint32_t reserved[2];
 } ppa_tuning;
 
 /* Counters of the last generate call (cuPPA_1.cu:84-87 timing CSV and the
@@ -92,6 +94,7 @@ typedef struct ppa_stats {
     uint64_t stalls;         /* polls that found the slot still empty */
     uint64_t claims;         /* work-claim atomics */
     uint64_t sleeps;         /* wavefront back-off sleeps */
+    uint64_t staged_rows;    /* rows published through the coalesced LDS path */
     uint64_t grid_blocks;    /* persistent workgroups launched */
     uint64_t block_threads;
     double kernel_ms;        /* device time of the generation kernel */
@@ -135,6 +138,101 @@ int ppa_ctx_sync(ppa_ctx *ctx, void *stream);
 int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_targets,
                          uint32_t *d_degree, uint64_t *d_hist, uint64_t hist_len,
                          uint64_t *d_max_degree, void *stream);
+
+/* ---- batched ensemble (no counterpart in the reference: it generates one
+ *      graph per process run; this is cuPPA::generateGraph over many graphs
+ *      in one persistent launch) ------------------------------------------- */
+/* Graph i writes d_targets[descs[i].targets_offset .. + n_i*d_i) and, if
+ * d_row_ptr is not NULL, d_row_ptr[descs[i].row_ptr_offset .. + n_i + 1)
+ * with values targets_offset_i + t*d_i (offsets into d_targets, so each
+ * graph's CSR indexes the concatenated array). Regions must not overlap.
+ * Every graph's output equals ppa_generate of that graph alone. */
+typedef struct ppa_batch ppa_batch;
+/* Uploads the descriptor table and captures reset + fill + generate into a
+ * hipGraph. The batch borrows the context: one launch in flight per context. */
+int ppa_batch_create(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, int32_t *d_targets,
+                     int64_t *d_row_ptr, ppa_batch **out);
+int ppa_batch_launch(ppa_batch *batch, void *stream);   /* replays the graph, does not wait */
+int ppa_batch_sync(ppa_batch *batch, void *stream);     /* waits and returns the kernel status */
+int ppa_batch_destroy(ppa_batch *batch);
+/* create + launch + sync + destroy */
+int ppa_generate_batch(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, int32_t *d_targets,
+                       int64_t *d_row_ptr, void *stream);
+
+/* ---- autotuner (replaces the online queue-pressure doubling of
+ *      cuPPA_2.cu:181-198) --------------------------------------------------- */
+/* Times candidate launch shapes on the given graphs (count == 1: the single
+ * graph generator; count > 1: the batch generator), sets the best on the
+ * context and returns it. Results are cached in the context by workload
+ * signature (n bucket, d, p bucket, batch size), so a repeated call with the
+ * same signature does not re-measure. Output buffers are overwritten. */
+int ppa_autotune(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, int32_t *d_targets,
+                 int64_t *d_row_ptr, void *stream, ppa_tuning *best);
+
+/* ---- serving loop (no counterpart in the reference) ------------------------
+ * Requests are queued, admitted first-in first-out against a device memory
+ * budget (each request's targets + row_ptr live in one pool allocated at
+ * creation), batched as they are admitted, and generated on up to `streams`
+ * concurrent persistent launches, so requests that arrive while others run
+ * overlap with them on the device. Results stay on the device until released.
+ * A request's output equals ppa_generate of the same graph alone; its row_ptr
+ * holds t*d (offsets into its own targets). Calls must not overlap (one host
+ * thread drives a server). */
+typedef struct ppa_server ppa_server;
+
+typedef struct ppa_server_config {
+    uint64_t budget_bytes; /* device pool for request outputs */
+    uint32_t streams;      /* concurrent launches (1..8, 0 = 4) */
+    uint32_t max_batch;    /* requests per launch (0 = 4096) */
+    uint64_t batch_nodes;  /* a launch closes once it holds this many nodes (0 = 2^24) */
+    uint64_t solo_nodes;   /* requests of at least this many nodes launch alone (0 = 2^24) */
+    uint32_t plain;        /* 1: one request per launch, one stream, single-graph kernel */
+    uint32_t reserved;
+} ppa_server_config;
+
+typedef struct ppa_request {
+    uint64_t id;           /* caller-chosen, unique among live requests */
+    uint64_t n;
+    uint32_t d;
+    uint32_t reserved;
+    uint64_t p_threshold;
+    uint64_t seed;
+} ppa_request;
+
+enum { PPA_REQ_QUEUED = 0, PPA_REQ_RUNNING = 1, PPA_REQ_DONE = 2, PPA_REQ_FAILED = 3, PPA_REQ_CANCELLED = 4 };
+
+typedef struct ppa_result {
+    int32_t state;         /* PPA_REQ_* */
+    int32_t status;        /* PPA_OK or the launch's error code */
+    int32_t *targets;      /* device, n*d, valid while DONE and not released */
+    int64_t *row_ptr;      /* device, n+1 */
+    uint64_t n;
+    uint32_t d;
+    uint32_t reserved;
+    double submit_ms;      /* host steady clock, ms since the server was created */
+    double start_ms;       /* launch enqueued */
+    double done_ms;        /* completion observed by ppa_server_step */
+} ppa_result;
+
+int ppa_server_create(ppa_ctx *ctx, const ppa_server_config *cfg, ppa_server **out);
+int ppa_server_destroy(ppa_server *srv);
+/* PPA_ERR_BUDGET if the request can never fit the budget; PPA_ERR_INVALID on a
+ * bad descriptor or a duplicate id. */
+int ppa_server_submit(ppa_server *srv, const ppa_request *req);
+/* Non-blocking progress: retires finished launches, admits queued requests
+ * and starts launches on free streams. Returns the number of requests that
+ * finished in this call (>= 0) or an error code. */
+int ppa_server_step(ppa_server *srv);
+int ppa_server_poll(ppa_server *srv, uint64_t id, ppa_result *out);
+int ppa_server_release(ppa_server *srv, uint64_t id);  /* frees a finished request's memory */
+int ppa_server_cancel(ppa_server *srv, uint64_t id);   /* only while still queued */
+int ppa_server_usage(ppa_server *srv, uint64_t *used_bytes, uint64_t *budget_bytes, uint64_t *queued,
+                     uint64_t *running);
+/* Replays a trace in real time: request i is submitted when the clock reaches
+ * arrival_ms[i] (ms from the start of the call) and released as soon as it
+ * finishes; latency_ms[i] = finish - arrival. makespan_ms = last finish. */
+int ppa_server_replay(ppa_server *srv, const ppa_request *reqs, const double *arrival_ms, uint64_t count,
+                      double *latency_ms, double *makespan_ms);
 
 /* ---- serial CPU path (`-A scm`): replaces SerialCopyModel::generateGraph
  *      (src/PA/SPA/SerialCopyModel3.cpp:11-107) ---------------------------- */

@@ -51,6 +51,9 @@ def test_cfg1_bit_exact(ctx, oracle_mod):
     dict(block_threads=128, chunk_nodes=128, max_blocks=1),
     dict(block_threads=64, chunk_nodes=64, max_blocks=3),
     dict(block_threads=256, chunk_nodes=512, blocks_per_cu=1),
+    dict(early_nodes=1),             # every row through the LDS-staged publication
+    dict(early_nodes=1 << 30),       # every slot published as accepted
+    dict(early_nodes=1024, chunk_nodes=64, max_blocks=2),
 ])
 def test_launch_shape_independent(ctx, oracle_mod, tuning):
     n, d, p, seed = 200_000, 8, 0.2, 99
