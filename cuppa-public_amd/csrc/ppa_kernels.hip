@@ -171,8 +171,11 @@ __global__ __launch_bounds__(256) void fill_batch_kernel(BatchFillArgs a, uint64
 
 // The generator. DMAX is the out-degree (single graph, BATCH = false) or
 // its upper bound (BATCH = true: every lane carries its graph's parameters).
+constexpr int kFlushPasses = 3;
+
 template <int DMAX, bool BATCH>
-__global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BATCH ? 4 : (DMAX <= 8 || DMAX == 12 ? 8 : (DMAX <= 11 ? 6 : 5)))))
+void generate_kernel(GenArgs a)
 {
     __shared__ __attribute__((aligned(16))) int32_t stage_lds[4][64 * DMAX];
     const uint32_t lane = lane_id();
@@ -201,8 +204,11 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
 #pragma unroll
     for (int i = 0; i < DMAX; ++i) val[i] = kEmpty;
 
-    uint32_t c_nodes = 0, c_slots = 0, c_draws = 0, c_retries = 0, c_pmiss = 0, c_polls = 0, c_stalls = 0,
-             c_claims = 0, c_sleeps = 0, c_staged = 0;
+    // Statistics: wavefront-uniform counters fed by ballots; per-lane only
+    // where the amount depends on the lane's out-degree.
+    uint32_t s_nodes = 0, s_retries = 0, s_pmiss = 0, s_polls = 0, s_stalls = 0, s_claims = 0, s_sleeps = 0,
+             s_staged = 0;
+    uint32_t c_draws = 0, c_slots = 0;
     uint32_t iter = 0, backoff = 1;
     int fail = 0;
 
@@ -214,11 +220,9 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
             if (wcur >= wend) {
                 const uint32_t size = (!BATCH && wlast < a.early_nodes) ? a.early_chunk : a.chunk;
                 uint64_t base = 0;
-                if (lane == 0) {
-                    base = atomicAdd(&a.scratch->claim, (unsigned long long)size);
-                    ++c_claims;
-                }
+                if (lane == 0) base = atomicAdd(&a.scratch->claim, (unsigned long long)size);
                 base = uniform64(base);
+                ++s_claims;
                 wlast = base;
                 if (BATCH) {
                     if (base >= a.work_base[a.count]) {
@@ -274,6 +278,7 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
         auto node_of = [&](uint64_t pair) -> int32_t {
             return (int32_t)(BATCH ? div_small(pair, dd, inv_d) : pair / (uint64_t)DMAX);
         };
+        bool polled = false, stalled = false;
         if (fresh) {
             // Attempt 0 of every slot at once; copy reads go out together.
 #pragma unroll
@@ -308,12 +313,12 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
             }
             if (have_src) {
                 got = load_coherent(G + src);
-                ++c_polls;
+                polled = true;
                 if (got != kEmpty) {
                     resolved = true;
                     have_src = false;
                 } else {
-                    ++c_stalls;
+                    stalled = true;
                 }
             }
             if (resolved) {
@@ -325,7 +330,7 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
         }
 
         // ---- accept slots in order: duplicate check against filled slots ----
-        bool finished = false;
+        bool finished = false, pmiss = false, retried = false;
         if (active && !need_draw && !have_src) {
             bool stop = false;
 #pragma unroll
@@ -336,14 +341,14 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
                         // attempt 0's copy source was still empty: re-derive it
                         need_draw = true;
                         stop = true;
-                        ++c_pmiss;
+                        pmiss = true;
                     } else {
                         bool dup = false;
 #pragma unroll
                         for (int i = 0; i < kk; ++i) dup |= (val[i] == c);
                         if (dup) {
                             ++r;
-                            ++c_retries;
+                            retried = true;
                             need_draw = true;
                             stop = true;
                             if (r >= kMaxRetry) fail = kErrRetry;
@@ -359,21 +364,28 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
             if (k == dd) {
                 active = false;
                 finished = true;
-                ++c_nodes;
-                c_slots += dd;
+                if (BATCH) c_slots += dd;
             }
         }
+        s_polls += (uint32_t)__popcll(__ballot(polled));
+        s_stalls += (uint32_t)__popcll(__ballot(stalled));
+        s_pmiss += (uint32_t)__popcll(__ballot(pmiss));
+        s_retries += (uint32_t)__popcll(__ballot(retried));
+        s_nodes += (uint32_t)__popcll(__ballot(finished));
 
         // ---- publish finished rows: coalesced through LDS (wavefront-uniform) ----
-        const bool want = finished && !early;
-        if (__ballot(want) != 0) {
-            const uint64_t key = want ? (((uint64_t)gid << 32) | t) : ~0ull;
+        // Each pass publishes the finished rows within 64 nodes of the smallest
+        // unpublished one, as whole contiguous sectors; rows left after the
+        // passes are stored by their own lanes.
+        bool pending = finished && !early;
+        for (int pass = 0; pass < kFlushPasses && __ballot(pending) != 0; ++pass) {
+            const uint64_t key = pending ? (((uint64_t)gid << 32) | t) : ~0ull;
             const uint64_t kmin = wave_min_u64(key);
-            const bool in_range = want && key - kmin < 64;
+            const bool in_range = pending && key - kmin < 64;
             // bit r set <=> staged row r (node t0 + r) is finished this iteration;
             // indexed by row, not by lane
             const uint64_t rmask = wave_or_u64(in_range ? 1ull << (key - kmin) : 0ull);
-            const uint32_t lead = (uint32_t)__builtin_ctzll(__ballot(want && key == kmin));
+            const uint32_t lead = (uint32_t)__builtin_ctzll(__ballot(pending && key == kmin));
             const uint32_t Ld = BATCH ? (uint32_t)__builtin_amdgcn_readlane((int)dd, (int)lead) : (uint32_t)DMAX;
             int32_t *const LG = BATCH ? (int32_t *)readlane64((uint64_t)G, lead) : a.G;
             const uint64_t t0 = kmin & 0xffffffffull;
@@ -382,8 +394,8 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
 #pragma unroll
                 for (int kk = 0; kk < DMAX; ++kk)
                     if (!BATCH || (uint32_t)kk < dd) stage[row * Ld + kk] = val[kk];
-                ++c_staged;
             }
+            s_staged += (uint32_t)__popcll(rmask);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -411,12 +423,23 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
                     }
                 }
             }
-            if (want && !in_range) {
+            __builtin_amdgcn_wave_barrier();
+            pending = pending && !in_range;
+        }
+        if (pending) {
+            if (!BATCH && DMAX % 2 == 0) {
+                // rows are 8-byte aligned: 8-byte agent-scope stores
+                uint64_t *row64 = reinterpret_cast<uint64_t *>(G + td);
+#pragma unroll
+                for (int kk = 0; kk < DMAX / 2; ++kk)
+                    __hip_atomic_store(row64 + kk,
+                                       ((uint64_t)(uint32_t)val[2 * kk + 1] << 32) | (uint32_t)val[2 * kk],
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
 #pragma unroll
                 for (int kk = 0; kk < DMAX; ++kk)
                     if (!BATCH || (uint32_t)kk < dd) store_coherent(G + td + kk, val[kk]);
             }
-            __builtin_amdgcn_wave_barrier();
         }
 
         // ---- failure, watchdog and back-off (wavefront-uniform) ----
@@ -438,7 +461,7 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
         }
         if (__ballot(progressed) == 0) {
             // Every lane of this wavefront is waiting on another node.
-            if (lane == 0) ++c_sleeps;
+            ++s_sleeps;
             for (uint32_t s = 0; s < backoff; ++s) __builtin_amdgcn_s_sleep(4);
             backoff = backoff < 32 ? backoff * 2 : 32;
         } else {
@@ -447,16 +470,17 @@ __global__ __launch_bounds__(256) void generate_kernel(GenArgs a)
     }
 
     // ---- statistics: one atomic per counter per wavefront ----
-    const uint32_t sums[10] = {wave_sum(c_nodes),  wave_sum(c_slots),  wave_sum(c_draws),  wave_sum(c_retries),
-                               wave_sum(c_pmiss),  wave_sum(c_polls),  wave_sum(c_stalls), wave_sum(c_claims),
-                               wave_sum(c_sleeps), wave_sum(c_staged)};
+    const uint32_t draws = wave_sum(c_draws);
+    const uint32_t slots = BATCH ? wave_sum(c_slots) : s_nodes * (uint32_t)DMAX;
     if (lane == 0) {
         unsigned long long *st = a.scratch->stats;
+        const uint32_t v[10] = {s_nodes, slots, draws, s_retries, s_pmiss, s_polls, s_stalls, s_claims, s_sleeps,
+                                s_staged};
         const int idx[10] = {kStatNodes,  kStatSlots,  kStatDraws,  kStatRetries, kStatPrefetchMisses,
                              kStatPolls,  kStatStalls, kStatClaims, kStatSleeps,  kStatStagedRows};
 #pragma unroll
         for (int i = 0; i < 10; ++i)
-            if (sums[i]) atomicAdd(&st[idx[i]], (unsigned long long)sums[i]);
+            if (v[i]) atomicAdd(&st[idx[i]], (unsigned long long)v[i]);
     }
 }
 

@@ -1,0 +1,41 @@
+"""The C++ host driver on the GPU (`-A cuppa` / `-A cuppa-adaptive`): same
+flags, stdout lines and files as the reference driver, graph equal to the
+oracle's."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUNNER = os.path.join(ROOT, "cuppa-public_amd", "bin", "ppa_runner")
+
+
+def _read_dd(path):
+    with open(path) as f:
+        assert f.readline().strip() == "Degree,Count"
+        return {int(a): int(b) for a, b in (line.strip().split(",") for line in f)}
+
+
+@pytest.mark.parametrize("algo", ["cuppa", "cuppa-adaptive"])
+def test_runner_gpu_outputs(tmp_path, oracle_mod, algo):
+    n, d, p, seed = 30_000, 6, 0.3, 77
+    r = subprocess.run([RUNNER, "-A", algo, "-n", str(n), "-x", str(d), "-a", str(p), "--seed", str(seed),
+                        "--show-dd", "1", "--dd-filename", "dd.csv", "-o", str(tmp_path) + "/",
+                        "--show-time-n-stages", "1", "--edges-file", str(tmp_path / "g.edges"), "-t", "128"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Computation Time: " in r.stdout and "Ended" in r.stdout
+    G, _, _ = oracle_mod.generate(n, d, oracle_mod.p_threshold(p), seed)
+    deg, _, _ = oracle_mod.degrees(n, d, G)
+    want = np.bincount(deg.astype(np.int64))
+    assert _read_dd(tmp_path / "dd.csv") == {i: int(c) for i, c in enumerate(want) if c}
+    edges = np.loadtxt(tmp_path / "g.edges", dtype=np.int64)
+    nclique = d * (d + 1) // 2
+    assert np.array_equal(edges[nclique:, 1], G[(d + 1) * d:])
+    csv = "StepwiseTiming-Original.csv" if algo == "cuppa" else "StepwiseTiming-Adaptive.csv"
+    lines = (tmp_path / csv).read_text().splitlines()
+    assert lines[0] == "Round,nBlocks,nThreads,Edges,MaxQ,Initialization Time,Computation Time"
+    assert len(lines) == 2 and lines[1].split(",")[3] == str(n * d)
