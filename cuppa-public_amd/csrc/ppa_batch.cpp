@@ -3,12 +3,16 @@
 //
 // The reference generates one graph per process and tunes online by doubling
 // threads and stage size while its LDS queues stay half empty
-// (cuPPA_2.cu:181-198). Here a batch of graphs is one work space: graph i
-// owns a chunk-aligned range of work indices, wavefronts claim chunks of that
-// space in ascending order (a chunk never spans two graphs), and the reset,
-// fill and generation kernels are captured once into a hipGraph that every
-// launch replays. The autotuner measures launch shapes on the caller's own
-// graphs and keeps the fastest per workload signature.
+// (cuPPA_2.cu:181-198). Here a batch of graphs is one work space: every
+// graph's nodes are cut into chunks, the chunks are handed out round by round
+// across the graphs (chunk r of every graph that has one, then chunk r+1,
+// ...), so each graph is claimed in ascending order -- the condition for
+// progress -- while thousands of graphs advance side by side with small
+// in-flight windows, instead of one small graph at a time being entirely in
+// flight and waiting on itself. The reset, fill and generation kernels are
+// captured once into a hipGraph that every launch replays. The autotuner
+// measures launch shapes on the caller's own graphs and keeps the fastest per
+// workload signature.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -32,7 +36,8 @@ namespace ppa {
 
 size_t batch_tables_bytes(uint64_t count)
 {
-    return count * sizeof(BatchGraph) + 3 * (count + 1) * sizeof(uint64_t);
+    return count * sizeof(BatchGraph) + (2 * (count + 1) + (count + 1) + 2 * count) * sizeof(uint64_t) +
+           count * sizeof(uint32_t);
 }
 
 BatchGraph graph_record(const ppa_graph_desc &dsc)
@@ -48,52 +53,100 @@ BatchGraph graph_record(const ppa_graph_desc &dsc)
     return gr;
 }
 
+namespace {
+
+struct TableView {
+    BatchGraph *graphs;
+    uint64_t *slot, *row, *seg_base, *seg_round, *seg_active;
+    uint32_t *order;
+};
+
+TableView view_of(char *base, uint64_t count, uint64_t nseg)
+{
+    TableView v;
+    v.graphs = (BatchGraph *)base;
+    v.slot = (uint64_t *)(base + count * sizeof(BatchGraph));
+    v.row = v.slot + (count + 1);
+    v.seg_base = v.row + (count + 1);
+    v.seg_round = v.seg_base + (nseg + 1);
+    v.seg_active = v.seg_round + nseg;
+    v.order = (uint32_t *)(v.seg_active + nseg);
+    return v;
+}
+
+}  // namespace
+
 void build_batch_tables(const std::vector<BatchGraph> &graphs, uint64_t chunk, BatchTables *out)
 {
     const uint64_t count = graphs.size();
-    out->count = count;
-    out->bytes.assign(batch_tables_bytes(count), 0);
-    char *base = out->bytes.data();
-    std::memcpy(base, graphs.data(), count * sizeof(BatchGraph));
-    auto *work = (uint64_t *)(base + count * sizeof(BatchGraph));
-    auto *slot = work + (count + 1);
-    auto *row = slot + (count + 1);
-    work[0] = slot[0] = row[0] = 0;
+    std::vector<uint64_t> chunks(count);
+    std::vector<uint32_t> order(count);
     out->work = 0;
     out->any_row_ptr = false;
     for (uint64_t i = 0; i < count; ++i) {
-        const BatchGraph &g = graphs[i];
-        const uint64_t w = g.n - g.d - 1;
+        const uint64_t w = graphs[i].n - graphs[i].d - 1;
         out->work += w;
-        work[i + 1] = work[i] + (w + chunk - 1) / chunk * chunk;
-        slot[i + 1] = slot[i] + g.n * g.d;
-        row[i + 1] = row[i] + (g.n + 1);
-        out->any_row_ptr |= g.row_ptr != nullptr;
+        chunks[i] = (w + chunk - 1) / chunk;
+        order[i] = (uint32_t)i;
+        out->any_row_ptr |= graphs[i].row_ptr != nullptr;
     }
-    out->total_slots = slot[count];
-    out->total_rows = out->any_row_ptr ? row[count] : 0;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return chunks[x] > chunks[y]; });
+    // segments: rounds [v_i, v_{i+1}) over the distinct positive chunk counts
+    std::vector<uint64_t> seg_base{0}, seg_round, seg_active;
+    uint64_t round = 0;
+    for (uint64_t i = count; i-- > 0;) {  // ascending chunk counts
+        const uint64_t c = chunks[order[i]];
+        if (c <= round) continue;
+        const uint64_t active = i + 1;  // graphs order[0..i] have more than `round` chunks
+        seg_round.push_back(round);
+        seg_active.push_back(active);
+        seg_base.push_back(seg_base.back() + (c - round) * active);
+        round = c;
+    }
+    const uint64_t nseg = seg_round.size();
+    out->count = count;
+    out->nseg = nseg;
+    out->bytes.assign(count * sizeof(BatchGraph) + (2 * (count + 1) + (nseg + 1) + 2 * nseg) * sizeof(uint64_t) +
+                          count * sizeof(uint32_t),
+                      0);
+    TableView v = view_of(out->bytes.data(), count, nseg);
+    std::memcpy(v.graphs, graphs.data(), count * sizeof(BatchGraph));
+    v.slot[0] = v.row[0] = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+        v.slot[i + 1] = v.slot[i] + graphs[i].n * graphs[i].d;
+        v.row[i + 1] = v.row[i] + (graphs[i].n + 1);
+    }
+    std::memcpy(v.seg_base, seg_base.data(), (nseg + 1) * sizeof(uint64_t));
+    if (nseg) {
+        std::memcpy(v.seg_round, seg_round.data(), nseg * sizeof(uint64_t));
+        std::memcpy(v.seg_active, seg_active.data(), nseg * sizeof(uint64_t));
+    }
+    std::memcpy(v.order, order.data(), count * sizeof(uint32_t));
+    out->total_slots = v.slot[count];
+    out->total_rows = out->any_row_ptr ? v.row[count] : 0;
 }
 
 hipError_t enqueue_batch(void *d_tables, const BatchTables &t, const LaunchShape &shape, DeviceScratch *scratch,
                          hipStream_t stream)
 {
-    char *base = (char *)d_tables;
-    auto *d_graphs = (BatchGraph *)base;
-    auto *d_work = (uint64_t *)(base + t.count * sizeof(BatchGraph));
-    auto *d_slot = d_work + (t.count + 1);
-    auto *d_row = d_slot + (t.count + 1);
+    TableView v = view_of((char *)d_tables, t.count, t.nseg);
     hipError_t e = launch_reset_scratch(scratch, 0, stream);
     if (e != hipSuccess) return e;
-    BatchFillArgs fa{d_graphs, d_slot, d_row, t.count};
+    BatchFillArgs fa{v.graphs, v.slot, v.row, t.count};
     if ((e = launch_fill_batch(fa, t.total_slots, t.total_rows, stream)) != hipSuccess) return e;
     if (t.work == 0) return hipSuccess;
     GenArgs ga{};
-    ga.graphs = d_graphs;
-    ga.work_base = d_work;
-    ga.count = t.count;
+    ga.graphs = v.graphs;
+    ga.order = v.order;
+    ga.seg_base = v.seg_base;
+    ga.seg_round = v.seg_round;
+    ga.seg_active = v.seg_active;
+    ga.nseg = t.nseg;
     ga.chunk = shape.chunk;
     ga.early_chunk = shape.chunk;
-    ga.early_nodes = shape.early_nodes;
+    // Round-robin claims keep a few chunks of each graph in flight, so only a
+    // graph's first rounds are dense enough to need slot-by-slot publication.
+    ga.early_nodes = std::min<uint64_t>(shape.early_nodes, 16ull * shape.chunk);
     ga.watchdog_ticks = shape.watchdog_ticks;
     ga.scratch = scratch;
     return launch_generate_batch(ga, shape.blocks, shape.threads, stream);

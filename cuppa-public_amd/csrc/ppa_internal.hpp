@@ -67,15 +67,18 @@ struct LaunchShape {
 int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *out);
 
 // Host image of a batch's device tables, packed as
-// BatchGraph[count] | work_base[count+1] | slot_base[count+1] | row_base[count+1].
+// BatchGraph[count] | slot_base[count+1] | row_base[count+1] | seg_base[nseg+1] |
+// seg_round[nseg] | seg_active[nseg] | order[count] (uint32).
 struct BatchTables {
     std::vector<char> bytes;
     uint64_t count = 0;
+    uint64_t nseg = 0;
     uint64_t work = 0;         // generated nodes
     uint64_t total_slots = 0;  // sum of n*d
     uint64_t total_rows = 0;   // sum of n+1 over graphs with a row_ptr
     bool any_row_ptr = false;
 };
+// Upper bound of the packed size for `count` graphs (nseg <= count).
 size_t batch_tables_bytes(uint64_t count);
 // graphs[i].G / row_ptr / targets_offset must be set; the rest is derived here.
 void build_batch_tables(const std::vector<BatchGraph> &graphs, uint64_t chunk, BatchTables *out);

@@ -195,6 +195,7 @@ void generate_kernel(GenArgs a)
     uint64_t t = 0;
     bool active = false, fresh = false, need_draw = false, have_src = false, early = false;
     uint32_t k = 0, r = 0;
+    uint32_t skip = 0, misses = 0;  // re-read back-off after a still-empty source
     uint64_t src = 0;
     int32_t *G = a.G;
     uint64_t T = a.T;
@@ -225,21 +226,27 @@ void generate_kernel(GenArgs a)
                 ++s_claims;
                 wlast = base;
                 if (BATCH) {
-                    if (base >= a.work_base[a.count]) {
+                    // global chunk index -> (segment, round, graph)
+                    const uint64_t c = base / size;
+                    if (c >= a.seg_base[a.nseg]) {
                         wdone = true;
                     } else {
-                        wgid = (uint32_t)find_segment(a.work_base, a.count, base);
+                        const uint64_t sg = find_segment(a.seg_base, a.nseg, c);
+                        const uint64_t off = c - a.seg_base[sg];
+                        const uint64_t act = a.seg_active[sg];
+                        const uint64_t round = a.seg_round[sg] + off / act;
+                        wgid = a.order[off % act];
                         const BatchGraph &gr = a.graphs[wgid];
-                        const uint64_t gend = a.work_base[wgid] + (gr.n - gr.d - 1);
-                        wcur = base;
-                        wend = base + size < gend ? base + size : gend;
+                        const uint64_t work = gr.n - gr.d - 1;
+                        wcur = round * size;
+                        wend = wcur + size < work ? wcur + size : work;
                         wG = gr.G;
                         wT = gr.T;
                         wk0 = gr.key0;
                         wk1 = gr.key1;
                         wd = gr.d;
                         winv = gr.inv_d;
-                        wtoff = (uint64_t)gr.d + 1 - a.work_base[wgid];
+                        wtoff = (uint64_t)gr.d + 1;
                     }
                 } else if (base >= a.n) {
                     wdone = true;
@@ -292,6 +299,8 @@ void generate_kernel(GenArgs a)
             c_draws += dd;
             k = 0;
             r = 0;
+            skip = 0;
+            misses = 0;
             need_draw = false;
             have_src = false;
             fresh = false;
@@ -311,14 +320,20 @@ void generate_kernel(GenArgs a)
                     have_src = true;
                 }
             }
-            if (have_src) {
+            if (have_src && skip > 0) {
+                --skip;
+            } else if (have_src) {
                 got = load_coherent(G + src);
                 polled = true;
                 if (got != kEmpty) {
                     resolved = true;
                     have_src = false;
+                    misses = 0;
                 } else {
                     stalled = true;
+                    // wait 0, 1, 2, 4, 8, 8, ... iterations before the next re-read
+                    skip = misses == 0 ? 0u : (1u << (misses < 4 ? misses - 1 : 3));
+                    ++misses;
                 }
             }
             if (resolved) {

@@ -49,11 +49,19 @@ struct GenArgs {
     uint64_t n;
     uint64_t T;
     uint32_t key0, key1;
-    // batch: graphs[i] owns work indices [work_base[i], work_base[i] + n_i - d_i - 1);
-    // work_base is a chunk-aligned exclusive prefix with count + 1 entries
+    // batch: chunks are handed out round by round -- chunk r of every graph
+    // that has more than r chunks, then chunk r+1, ... -- so each graph's
+    // chunks are claimed in ascending order while many graphs progress side
+    // by side. Rounds with the same set of active graphs form a segment:
+    // segment s starts at global chunk seg_base[s] and round seg_round[s], and
+    // its rounds cover graphs order[0 .. seg_active[s]) (graphs sorted by
+    // chunk count, descending). seg_base has nseg + 1 entries.
     const BatchGraph *graphs;
-    const uint64_t *work_base;
-    uint64_t count;
+    const uint32_t *order;
+    const uint64_t *seg_base;
+    const uint64_t *seg_round;
+    const uint64_t *seg_active;
+    uint64_t nseg;
     // common
     uint32_t chunk;           // work claimed per wavefront claim (multiple of 64)
     uint32_t early_chunk;     // claim size below early_nodes (single graph)

@@ -171,6 +171,11 @@ int launch(ppa_server *s, Slot &sl, const std::vector<uint64_t> &ids)
     LaunchShape shape;
     int rc = resolve_shape(s->ctx, s->cfg.plain ? graphs[0].d : 0, work, &shape);
     if (rc) return rc;
+    // A persistent launch keeps its workgroups until its work is claimed; while
+    // other launches are in flight, take a share of the GPU so they overlap.
+    uint32_t busy = 0;
+    for (const auto &o : s->slots) busy += o.busy ? 1u : 0u;
+    if (busy) shape.blocks = std::max<uint32_t>(1u, shape.blocks / (busy + 1));
     if (s->cfg.plain) {
         const Request &r = s->reqs[ids[0]];
         PPA_HIP(launch_reset_scratch(sl.scratch, (uint64_t)r.req.d + 1, sl.stream));

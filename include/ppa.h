@@ -78,7 +78,8 @@ typedef struct ppa_tuning {
     uint32_t max_blocks;     /* cap on the persistent grid (0 = no cap) */
     uint32_t watchdog_ms;    /* kernel deadline (0 = default 60000) */
     uint32_t early_nodes;    /* nodes below this index publish each slot as it is accepted and
-                                are claimed one per lane (0 = default 2^22, 1 = none) */
+                                are claimed one per lane (0 = default 2^22, 1 = none); batches
+                                apply at most 16 chunks of it per graph */
     uint32_t reserved[2];
 } ppa_tuning;
 
