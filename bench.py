@@ -200,9 +200,17 @@ def run_serve(args):
         srv.close()
         lat = np.array(lat)
         e = edges_all[:len(reqs)].sum()
+        sizes = np.array([n for _, n, _, _, _ in reqs])
+        by_size = {}
+        for lo, hi, name in ((0, 1e5, "n<1e5"), (1e5, 1e6, "1e5<=n<1e6"), (1e6, 1e7, "1e6<=n<1e7"), (1e7, 1e9, "n>=1e7")):
+            m = (sizes >= lo) & (sizes < hi)
+            if m.any():
+                by_size[name] = {"count": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50)),
+                                 "p99_ms": float(np.percentile(lat[m], 99))}
         return {"rate_per_s": rate, "requests": len(reqs), "p50_ms": float(np.percentile(lat, 50)),
-                "p99_ms": float(np.percentile(lat, 99)), "makespan_ms": makespan,
-                "req_per_s": len(reqs) / (makespan * 1e-3), "edges_per_s": e / (makespan * 1e-3)}
+                "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()), "makespan_ms": makespan,
+                "req_per_s": len(reqs) / (makespan * 1e-3), "edges_per_s": e / (makespan * 1e-3),
+                "by_size": by_size}
 
     # plain: saturation throughput on a prefix, then 0.7x of it
     np_plain = min(args.serve_plain_count, len(trace))
@@ -211,7 +219,7 @@ def run_serve(args):
     plain = replay(True, trace[:np_plain], plain_rate)
     # served: increasing arrival rates; each run covers at most ~10 s of arrivals
     served = []
-    for mult in (1, 2, 5, 10, 20, 50, 100):
+    for mult in (1, 1.5, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64):
         rate = plain_rate * mult
         count = int(min(len(trace), max(np_plain, rate * 10)))
         res = replay(False, trace[:count], rate)

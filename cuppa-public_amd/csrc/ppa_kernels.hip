@@ -320,7 +320,9 @@ void generate_kernel(GenArgs a)
                     have_src = true;
                 }
             }
-            if (have_src && skip > 0) {
+            if (BATCH && have_src && skip > 0) {
+                // batches only: thousands of graphs' early rounds would
+                // otherwise flood memory with re-reads
                 --skip;
             } else if (have_src) {
                 got = load_coherent(G + src);

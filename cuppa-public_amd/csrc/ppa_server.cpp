@@ -168,15 +168,23 @@ int launch(ppa_server *s, Slot &sl, const std::vector<uint64_t> &ids)
         graphs.push_back(g);
         work += r.req.n - r.req.d - 1;
     }
+    // One request: the single-graph generator (compile-time d, full occupancy).
+    // Several: the batch generator over all of them.
+    const bool single = ids.size() == 1;
     LaunchShape shape;
-    int rc = resolve_shape(s->ctx, s->cfg.plain ? graphs[0].d : 0, work, &shape);
+    int rc = resolve_shape(s->ctx, single ? graphs[0].d : 0, work, &shape);
     if (rc) return rc;
-    // A persistent launch keeps its workgroups until its work is claimed; while
-    // other launches are in flight, take a share of the GPU so they overlap.
-    uint32_t busy = 0;
-    for (const auto &o : s->slots) busy += o.busy ? 1u : 0u;
-    if (busy) shape.blocks = std::max<uint32_t>(1u, shape.blocks / (busy + 1));
-    if (s->cfg.plain) {
+    if (!s->cfg.plain) {
+        // A persistent launch keeps its workgroups until its work is claimed.
+        // Large requests take at most 3/4 of the GPU and batches of small ones
+        // at most 1/4, so the two kinds overlap instead of queueing.
+        LaunchShape full;
+        if ((rc = resolve_shape(s->ctx, single ? graphs[0].d : 0, ~0ull >> 1, &full)) != PPA_OK) return rc;
+        const uint32_t cap = single && work + graphs[0].d + 1 >= s->cfg.solo_nodes ? full.blocks * 3 / 4
+                                                                                    : full.blocks / 4;
+        shape.blocks = std::max<uint32_t>(1u, std::min(shape.blocks, cap));
+    }
+    if (single) {
         const Request &r = s->reqs[ids[0]];
         PPA_HIP(launch_reset_scratch(sl.scratch, (uint64_t)r.req.d + 1, sl.stream));
         FillArgs fa{graphs[0].G, graphs[0].row_ptr, r.req.n, r.req.d, 0};
@@ -236,8 +244,8 @@ int ppa_server_create(ppa_ctx *ctx, const ppa_server_config *cfg, ppa_server **o
     if (s->cfg.streams > 8) s->cfg.streams = 8;
     if (s->cfg.plain) s->cfg.streams = 1;
     if (!s->cfg.max_batch) s->cfg.max_batch = 4096;
-    if (!s->cfg.batch_nodes) s->cfg.batch_nodes = 1ull << 24;
-    if (!s->cfg.solo_nodes) s->cfg.solo_nodes = 1ull << 24;
+    if (!s->cfg.batch_nodes) s->cfg.batch_nodes = 1ull << 22;
+    if (!s->cfg.solo_nodes) s->cfg.solo_nodes = 1ull << 20;
     const uint64_t budget = s->cfg.budget_bytes / kGranule * kGranule;
     if (hipMalloc(&s->pool_base, budget) != hipSuccess) {
         destroy_server(s);

@@ -185,8 +185,8 @@ typedef struct ppa_server_config {
     uint64_t budget_bytes; /* device pool for request outputs */
     uint32_t streams;      /* concurrent launches (1..8, 0 = 4) */
     uint32_t max_batch;    /* requests per launch (0 = 4096) */
-    uint64_t batch_nodes;  /* a launch closes once it holds this many nodes (0 = 2^24) */
-    uint64_t solo_nodes;   /* requests of at least this many nodes launch alone (0 = 2^24) */
+    uint64_t batch_nodes;  /* a launch closes once it holds this many nodes (0 = 2^22) */
+    uint64_t solo_nodes;   /* requests of at least this many nodes launch alone (0 = 2^20) */
     uint32_t plain;        /* 1: one request per launch, one stream, single-graph kernel */
     uint32_t reserved;
 } ppa_server_config;
