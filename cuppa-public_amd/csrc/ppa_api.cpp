@@ -6,6 +6,7 @@
 // loop (cuPPA_1.cu:112-226) becomes one persistent launch, and the
 // printf-and-continue error handling (cuPPA_1.cu:42-46,106-110) becomes
 // status codes.
+#include <algorithm>
 #include <cstring>
 #include <new>
 
@@ -17,6 +18,7 @@ bool tuning_valid(const ppa_tuning &t)
 {
     if (t.block_threads && (t.block_threads % 64 != 0 || t.block_threads > 256)) return false;
     if (t.chunk_nodes && (t.chunk_nodes % 64 != 0 || t.chunk_nodes > (1u << 20))) return false;
+    if (t.sleep_cap > 64) return false;
     return true;
 }
 
@@ -31,8 +33,13 @@ int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *ou
     if (per_cu < 1) per_cu = 1;
     if (tu.blocks_per_cu && (int)tu.blocks_per_cu < per_cu) per_cu = (int)tu.blocks_per_cu;
     uint64_t g = (uint64_t)per_cu * (uint64_t)ctx->num_cus;
-    // no more wavefronts than there are chunks of work
-    const uint64_t waves_needed = (work + ch - 1) / ch;
+    // Single graphs: about one lane per 8*d generated nodes, at least 16
+    // wavefronts. Dependency chains, not lanes, bound a small graph's run time
+    // (a node's copy source is uniform over the nodes before it), and every
+    // lane beyond that only adds re-reads of still-empty slots. Batches: no
+    // more wavefronts than claim chunks.
+    const uint64_t waves_needed =
+        d ? std::max<uint64_t>(16, (work + 512ull * d - 1) / (512ull * d)) : (work + ch - 1) / ch;
     const uint64_t blocks_needed = (waves_needed + th / 64 - 1) / (th / 64);
     if (g > blocks_needed) g = blocks_needed;
     if (tu.max_blocks && g > tu.max_blocks) g = tu.max_blocks;
@@ -43,6 +50,11 @@ int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *ou
     out->early_nodes = tu.early_nodes ? tu.early_nodes : kDefaultEarlyNodes;
     const uint32_t wd = tu.watchdog_ms ? tu.watchdog_ms : kDefaultWatchdogMs;
     out->watchdog_ticks = (uint64_t)wd * (uint64_t)ctx->wall_clock_khz;
+    // A waiting lane re-reads its source once per loop pass. A few hundred
+    // wavefronts can do so at full speed; a full GPU of them would flood
+    // memory with re-reads, so its all-waiting wavefronts sleep longer.
+    const uint64_t waves = g * (th / 64);
+    out->sleep_cap = tu.sleep_cap ? tu.sleep_cap : (uint32_t)std::min<uint64_t>(32, std::max<uint64_t>(1, waves / 256));
     return PPA_OK;
 }
 
@@ -211,6 +223,7 @@ int ppa_generate_async(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targ
         ga.early_chunk = kEarlyChunk;
         ga.early_nodes = shape.early_nodes;
         ga.watchdog_ticks = shape.watchdog_ticks;
+        ga.sleep_cap = shape.sleep_cap;
         ga.scratch = ctx->scratch;
         PPA_HIP(launch_generate(d, ga, shape.blocks, shape.threads, stream));
     }

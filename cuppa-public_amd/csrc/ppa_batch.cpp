@@ -148,6 +148,7 @@ hipError_t enqueue_batch(void *d_tables, const BatchTables &t, const LaunchShape
     // graph's first rounds are dense enough to need slot-by-slot publication.
     ga.early_nodes = std::min<uint64_t>(shape.early_nodes, 16ull * shape.chunk);
     ga.watchdog_ticks = shape.watchdog_ticks;
+    ga.sleep_cap = shape.sleep_cap;
     ga.scratch = scratch;
     return launch_generate_batch(ga, shape.blocks, shape.threads, stream);
 }

@@ -46,6 +46,18 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
     return v;
 }
 
+__device__ __forceinline__ uint64_t wave_sum64(uint32_t v)
+{
+    uint64_t x = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)x, off, kWave);
+        const uint32_t hi = __shfl_xor((uint32_t)(x >> 32), off, kWave);
+        x += ((uint64_t)hi << 32) | lo;
+    }
+    return x;
+}
+
 // floor(R / d) for a runtime divisor 1 <= d <= 2^16 and R < 2^52: a double
 // estimate off by at most one, corrected with an exact integer remainder.
 __device__ __forceinline__ uint64_t div_small(uint64_t R, uint32_t d, double inv_d)

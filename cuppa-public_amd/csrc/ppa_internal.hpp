@@ -60,6 +60,7 @@ struct LaunchShape {
     uint32_t blocks = 0, threads = 0, chunk = 0;
     uint64_t early_nodes = 0;
     uint64_t watchdog_ticks = 0;
+    uint32_t sleep_cap = 1;
 };
 
 // Grid, workgroup and claim sizes for `work` generated nodes with out-degree

@@ -26,8 +26,16 @@
 //     issues their d copy reads together (d-way memory-level parallelism);
 //     the duplicate check then walks the slots in order from registers. Only a
 //     rejected attempt (duplicate) or an empty source slot costs another
-//     round trip. The reference instead parks placeholders -(v*M+k) in a
-//     per-thread LDS queue and spins (cuPPA_RRP_1.cu:97-185).
+//     round trip. An empty source is kept in its slot as -2 - source, and a
+//     node's empty sources are re-read together (the first one every pass,
+//     the others every fourth), so a node waits for its slowest source, not
+//     for the sum of its sources' waits. The reference instead parks
+//     placeholders -(v*M+k) in a per-thread LDS queue and spins
+//     (cuPPA_RRP_1.cu:97-185).
+//   * A wavefront whose lanes all wait sleeps with a doubling back-off capped
+//     per launch: short for small grids, whose dependency chains set their
+//     run time, and long for a full GPU, where re-reads would crowd out the
+//     copy reads.
 //   * Finished rows are published per loop iteration through LDS: the rows
 //     finished by a wavefront within 64 consecutive nodes are transposed so
 //     that each store instruction writes whole, contiguous 32-byte sectors
@@ -46,6 +54,15 @@ namespace {
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
 constexpr int kBufferFlags = 0x00020000;  // gfx9 buffer descriptor word 3 (32-bit data format)
 constexpr int kSc1 = 16;                  // buffer-store cache policy: agent scope (sc1)
+
+// A slot whose attempt-0 copy source was still empty holds -2 - source until
+// the source is final (-1 keeps meaning "no value"); sources at or beyond
+// kPendingLimit are re-derived instead.
+constexpr uint64_t kPendingLimit = 0x7ffffffeull;
+constexpr int32_t kFarPair = 0x7fffffff;  // a copy source at or beyond kPendingLimit
+constexpr int kWaitVm0 = 0x0F70;          // s_waitcnt vmcnt(0), other counters untouched
+__device__ __forceinline__ int32_t pending_code(uint64_t src) { return (int32_t)(-2 - (int64_t)src); }
+__device__ __forceinline__ uint64_t pending_src(int32_t v) { return (uint64_t)(uint32_t)(-2 - v); }
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(void *base, uint32_t bytes)
 {
@@ -173,18 +190,31 @@ __global__ __launch_bounds__(256) void fill_batch_kernel(BatchFillArgs a, uint64
 // its upper bound (BATCH = true: every lane carries its graph's parameters).
 constexpr int kFlushPasses = 3;
 
+// Wavefronts per SIMD each out-degree is compiled for: the most whose register
+// budget (512 / waves) holds the kernel without spills.
+constexpr int waves_per_simd(int dmax, bool batch)
+{
+    return batch ? 4
+                 : (dmax <= 5 || dmax == 8) ? 8
+                 : (dmax <= 7 || dmax == 12) ? 6
+                 : (dmax <= 11 || dmax == 16) ? 5
+                 : 4;
+}
+
 template <int DMAX, bool BATCH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BATCH ? 4 : (DMAX <= 8 || DMAX == 12 ? 8 : (DMAX <= 11 ? 6 : 5)))))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_simd(DMAX, BATCH))))
 void generate_kernel(GenArgs a)
 {
     __shared__ __attribute__((aligned(16))) int32_t stage_lds[4][64 * DMAX];
+    __shared__ int32_t info_lds[4][64 * DMAX];
     const uint32_t lane = lane_id();
     int32_t *const stage = stage_lds[threadIdx.x >> 6];
+    int32_t *const info = info_lds[threadIdx.x >> 6];
     const uint64_t start_clock = wall_clock64();
 
     // Wavefront-uniform claim window [wcur, wend) of node (or work) indices,
     // and for batches the window's graph.
-    uint64_t wcur = 0, wend = 0, wlast = 0;
+    uint64_t wcur = 0, wend = 0, wlast = 0, fresh_base = 0;
     bool wdone = false;
     int32_t *wG = a.G;
     uint64_t wT = a.T, wtoff = 0;
@@ -207,9 +237,8 @@ void generate_kernel(GenArgs a)
 
     // Statistics: wavefront-uniform counters fed by ballots; per-lane only
     // where the amount depends on the lane's out-degree.
-    uint32_t s_nodes = 0, s_retries = 0, s_pmiss = 0, s_polls = 0, s_stalls = 0, s_claims = 0, s_sleeps = 0,
-             s_staged = 0;
-    uint32_t c_draws = 0, c_slots = 0;
+    uint32_t s_nodes = 0, s_retries = 0, s_claims = 0, s_sleeps = 0, s_staged = 0;
+    uint32_t c_draws = 0, c_slots = 0, c_pmiss = 0, c_polls = 0, c_stalls = 0;
     uint32_t iter = 0, backoff = 1;
     int fail = 0;
 
@@ -258,6 +287,7 @@ void generate_kernel(GenArgs a)
             if (!wdone) {
                 const uint64_t avail = wend - wcur;
                 const uint32_t rank = rank_below(idle_mask);
+                fresh_base = wcur + wtoff;
                 if (!active && rank < avail) {
                     t = wcur + rank + wtoff;
                     active = true;
@@ -280,22 +310,72 @@ void generate_kernel(GenArgs a)
         }
         if (wdone && __ballot(active) == 0) break;
 
+        // ---- attempt 0 of the nodes assigned this pass, over the whole wavefront ----
+        // The f lanes assigned this pass hold the consecutive nodes
+        // fresh_base + 0 .. f-1 of one claim window. Lane a % 64 computes pair
+        // a = i*d + kk (node i, slot kk) in pass a / 64 -- ceil(f*d / 64)
+        // Philox passes where per-lane draws would cost d. A copy read goes
+        // straight into word a of this wavefront's LDS stage (global_load_lds:
+        // no register waits on it, so every read of every pass is in flight
+        // together); word a of `info` keeps ~node for a direct draw, else the
+        // source pair (kFarPair when it cannot be kept as a pending code).
+        const uint64_t fresh_mask = __ballot(fresh);
+        if (fresh_mask != 0) {
+            const uint32_t fd = BATCH ? wd : (uint32_t)DMAX;
+            const uint32_t total = (uint32_t)__popcll(fresh_mask) * fd;
+            const uint32_t magic = (65536u + fd - 1u) / fd;  // exact a / fd for a < 1024, fd <= 16
+#pragma unroll 1
+            for (uint32_t p = 0; p * 64u < total; ++p) {
+                const uint32_t ai = p * 64u + lane;
+                if (ai < total) {
+                    const uint32_t i = BATCH ? (ai * magic) >> 16 : ai / (uint32_t)DMAX;
+                    const uint32_t kk = ai - i * fd;
+                    const uint64_t tt = fresh_base + i;
+                    const Attempt at = attempt(wk0, wk1, wT, tt, kk, 0u, tt * fd);
+                    if (at.direct) {
+                        info[ai] = ~(int32_t)(BATCH ? div_small(at.pair, fd, winv) : at.pair / (uint64_t)DMAX);
+                    } else {
+                        info[ai] = at.pair < kPendingLimit ? (int32_t)at.pair : kFarPair;
+                        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(wG + at.pair),
+                                                         (__attribute__((address_space(3))) void *)(stage + p * 64u),
+                                                         4, 0, 0);
+                    }
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (fresh) {
+                const uint32_t first = rank_below(fresh_mask) * fd;
+#pragma unroll
+                for (int kk = 0; kk < DMAX; ++kk) {
+                    if (!BATCH || (uint32_t)kk < fd) {
+                        const int32_t inf = info[first + kk];
+                        int32_t v = ~inf;
+                        if (inf >= 0) {
+                            v = stage[first + kk];
+                            // slot values are >= -1; anything else cannot be a read
+                            if (v <= kEmpty) {
+                                // an empty source is remembered in the slot itself
+                                ++c_pmiss;
+                                v = inf != kFarPair ? pending_code((uint32_t)inf) : kEmpty;
+                            }
+                        }
+                        val[kk] = v;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+
         const uint32_t dd = BATCH ? d : (uint32_t)DMAX;
         const uint64_t td = t * (uint64_t)dd;
         auto node_of = [&](uint64_t pair) -> int32_t {
             return (int32_t)(BATCH ? div_small(pair, dd, inv_d) : pair / (uint64_t)DMAX);
         };
-        bool polled = false, stalled = false;
         if (fresh) {
-            // Attempt 0 of every slot at once; copy reads go out together.
-#pragma unroll
-            for (int kk = 0; kk < DMAX; ++kk) {
-                if (!BATCH || (uint32_t)kk < dd) {
-                    const Attempt at = attempt(k0, k1, T, t, (uint32_t)kk, 0u, td);
-                    if (at.direct) val[kk] = node_of(at.pair);
-                    else val[kk] = __builtin_nontemporal_load(G + at.pair);
-                }
-            }
+            // attempt 0 of every slot arrived through the stage above
             c_draws += dd;
             k = 0;
             r = 0;
@@ -320,20 +400,54 @@ void generate_kernel(GenArgs a)
                     have_src = true;
                 }
             }
+            // Coherent re-reads, one round trip per pass: the source of a
+            // redrawn slot k (row 0 of the LDS info area), and the attempt-0
+            // sources still empty -- slot k every pass (batches: with a
+            // back-off), the later ones every fourth pass -- in row kk of the
+            // LDS stage. Acceptance stays in slot order, but the waits of a
+            // node's slots overlap. Agent-scope global_load_lds: no register
+            // holds a read in flight.
+            bool lpoll = false;
             if (BATCH && have_src && skip > 0) {
                 // batches only: thousands of graphs' early rounds would
                 // otherwise flood memory with re-reads
                 --skip;
             } else if (have_src) {
-                got = load_coherent(G + src);
-                polled = true;
-                if (got != kEmpty) {
+                lpoll = true;
+            }
+            const bool wide = (iter & 3u) == 0u;
+            const bool head_ok = !(BATCH && skip > 0);
+            uint32_t pmask = 0;
+#pragma unroll
+            for (int kk = 0; kk < DMAX; ++kk)
+                if ((uint32_t)kk >= k && (!BATCH || (uint32_t)kk < dd) && val[kk] < kEmpty &&
+                    ((uint32_t)kk == k ? head_ok : wide))
+                    pmask |= 1u << kk;
+            if (__ballot(lpoll || pmask != 0) != 0) {
+                if (lpoll)
+                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(G + src),
+                                                     (__attribute__((address_space(3))) void *)info, 4, 0, kSc1);
+#pragma unroll
+                for (int kk = 0; kk < DMAX; ++kk) {
+                    if ((pmask >> kk) & 1u)
+                        __builtin_amdgcn_global_load_lds(
+                            (const __attribute__((address_space(1))) void *)(G + pending_src(val[kk])),
+                            (__attribute__((address_space(3))) void *)(stage + kk * 64), 4, 0, kSc1);
+                }
+                c_polls += (uint32_t)__popc(pmask) + (lpoll ? 1u : 0u);
+                __builtin_amdgcn_s_waitcnt(kWaitVm0);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            if (lpoll) {
+                const int32_t v = info[lane];
+                if (v > kEmpty) {
+                    got = v;
                     resolved = true;
                     have_src = false;
                     misses = 0;
                 } else {
-                    stalled = true;
-                    // wait 0, 1, 2, 4, 8, 8, ... iterations before the next re-read
+                    ++c_stalls;
+                    // wait 0, 1, 2, 4, 8, 8, ... passes before the next re-read
                     skip = misses == 0 ? 0u : (1u << (misses < 4 ? misses - 1 : 3));
                     ++misses;
                 }
@@ -344,21 +458,46 @@ void generate_kernel(GenArgs a)
                 for (int kk = 0; kk < DMAX; ++kk)
                     if ((uint32_t)kk == k) val[kk] = got;
             }
+#pragma unroll
+            for (int kk = 0; kk < DMAX; ++kk) {
+                if ((uint32_t)kk >= k && (!BATCH || (uint32_t)kk < dd) && val[kk] < kEmpty) {
+                    const bool head = (uint32_t)kk == k;
+                    const int32_t got_p = (pmask >> kk) & 1u ? stage[kk * 64 + lane] : kEmpty;
+                    if (got_p > kEmpty) {
+                        val[kk] = got_p;
+                        progressed = true;
+                        if (head) misses = 0;
+                    } else if (head) {
+                        if (BATCH && skip > 0) {
+                            --skip;
+                        } else {
+                            ++c_stalls;
+                            skip = misses == 0 ? 0u : (1u << (misses < 4 ? misses - 1 : 3));
+                            ++misses;
+                        }
+                    } else if (wide) {
+                        ++c_stalls;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
         }
 
         // ---- accept slots in order: duplicate check against filled slots ----
-        bool finished = false, pmiss = false, retried = false;
+        bool finished = false, retried = false;
         if (active && !need_draw && !have_src) {
             bool stop = false;
 #pragma unroll
             for (int kk = 0; kk < DMAX; ++kk) {
                 if (!stop && (uint32_t)kk == k && (!BATCH || (uint32_t)kk < dd)) {
                     const int32_t c = val[kk];
-                    if (c == kEmpty) {
-                        // attempt 0's copy source was still empty: re-derive it
+                    if (c < kEmpty) {
+                        // attempt 0's copy source is still empty: keep re-reading it
+                        stop = true;
+                    } else if (c == kEmpty) {
+                        // an empty source too far out to remember: re-derive it
                         need_draw = true;
                         stop = true;
-                        pmiss = true;
                     } else {
                         bool dup = false;
 #pragma unroll
@@ -384,9 +523,6 @@ void generate_kernel(GenArgs a)
                 if (BATCH) c_slots += dd;
             }
         }
-        s_polls += (uint32_t)__popcll(__ballot(polled));
-        s_stalls += (uint32_t)__popcll(__ballot(stalled));
-        s_pmiss += (uint32_t)__popcll(__ballot(pmiss));
         s_retries += (uint32_t)__popcll(__ballot(retried));
         s_nodes += (uint32_t)__popcll(__ballot(finished));
 
@@ -480,7 +616,7 @@ void generate_kernel(GenArgs a)
             // Every lane of this wavefront is waiting on another node.
             ++s_sleeps;
             for (uint32_t s = 0; s < backoff; ++s) __builtin_amdgcn_s_sleep(4);
-            backoff = backoff < 32 ? backoff * 2 : 32;
+            backoff = backoff * 2 < a.sleep_cap ? backoff * 2 : a.sleep_cap;
         } else {
             backoff = 1;
         }
@@ -489,9 +625,10 @@ void generate_kernel(GenArgs a)
     // ---- statistics: one atomic per counter per wavefront ----
     const uint32_t draws = wave_sum(c_draws);
     const uint32_t slots = BATCH ? wave_sum(c_slots) : s_nodes * (uint32_t)DMAX;
+    const uint64_t pmiss = wave_sum64(c_pmiss), polls = wave_sum64(c_polls), stalls = wave_sum64(c_stalls);
     if (lane == 0) {
         unsigned long long *st = a.scratch->stats;
-        const uint32_t v[10] = {s_nodes, slots, draws, s_retries, s_pmiss, s_polls, s_stalls, s_claims, s_sleeps,
+        const uint64_t v[10] = {s_nodes, slots, draws, s_retries, pmiss, polls, stalls, s_claims, s_sleeps,
                                 s_staged};
         const int idx[10] = {kStatNodes,  kStatSlots,  kStatDraws,  kStatRetries, kStatPrefetchMisses,
                              kStatPolls,  kStatStalls, kStatClaims, kStatSleeps,  kStatStagedRows};

@@ -68,6 +68,8 @@ struct GenArgs {
     uint64_t early_nodes;     // below this node index every slot is published as accepted
     uint64_t watchdog_ticks;  // wall-clock ticks a wavefront may run
     DeviceScratch *scratch;
+    uint32_t sleep_cap;       // longest wavefront back-off, in 256-clock sleeps (>= 1)
+    uint32_t pad;
 };
 
 struct FillArgs {

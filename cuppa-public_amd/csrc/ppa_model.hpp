@@ -43,10 +43,11 @@ PPA_HD Draw philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t 
 {
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
-        const uint32_t lo_a = 0xD2511F53u * c0;
-        const uint32_t hi_a = mul_hi32(0xD2511F53u, c0);
-        const uint32_t lo_b = 0xCD9E8D57u * c2;
-        const uint32_t hi_b = mul_hi32(0xCD9E8D57u, c2);
+        // one 32x32->64 product per multiplier (v_mad_u64_u32 on gfx950)
+        const uint64_t pa = (uint64_t)0xD2511F53u * c0;
+        const uint64_t pb = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t lo_a = (uint32_t)pa, hi_a = (uint32_t)(pa >> 32);
+        const uint32_t lo_b = (uint32_t)pb, hi_b = (uint32_t)(pb >> 32);
         const uint32_t n0 = hi_b ^ c1 ^ k0;
         const uint32_t n2 = hi_a ^ c3 ^ k1;
         c0 = n0;

@@ -200,6 +200,7 @@ int launch(ppa_server *s, Slot &sl, const std::vector<uint64_t> &ids)
             ga.early_chunk = kEarlyChunk;
             ga.early_nodes = shape.early_nodes;
             ga.watchdog_ticks = shape.watchdog_ticks;
+            ga.sleep_cap = shape.sleep_cap;
             ga.scratch = sl.scratch;
             PPA_HIP(launch_generate(r.req.d, ga, shape.blocks, shape.threads, sl.stream));
         }
@@ -240,8 +241,8 @@ int ppa_server_create(ppa_ctx *ctx, const ppa_server_config *cfg, ppa_server **o
     if (!s) return PPA_ERR_NOMEM;
     s->ctx = ctx;
     s->cfg = *cfg;
-    if (!s->cfg.streams) s->cfg.streams = 4;
-    if (s->cfg.streams > 8) s->cfg.streams = 8;
+    if (!s->cfg.streams) s->cfg.streams = 16;
+    if (s->cfg.streams > 16) s->cfg.streams = 16;
     if (s->cfg.plain) s->cfg.streams = 1;
     if (!s->cfg.max_batch) s->cfg.max_batch = 4096;
     if (!s->cfg.batch_nodes) s->cfg.batch_nodes = 1ull << 22;
@@ -423,7 +424,7 @@ int ppa_server_usage(ppa_server *s, uint64_t *used, uint64_t *budget, uint64_t *
 }
 
 int ppa_server_replay(ppa_server *s, const ppa_request *reqs, const double *arrival_ms, uint64_t count,
-                      double *latency_ms, double *makespan_ms)
+                      double *latency_ms, double *wait_ms, double *makespan_ms)
 {
     if (!s || (count && (!reqs || !arrival_ms || !latency_ms))) return PPA_ERR_INVALID;
     std::unordered_map<uint64_t, uint64_t> index;
@@ -452,6 +453,7 @@ int ppa_server_replay(ppa_server *s, const ppa_request *reqs, const double *arri
                     if (r.state == PPA_REQ_FAILED) return r.status;
                     const uint64_t i = index[id];
                     latency_ms[i] = r.done_ms - (base + arrival_ms[i]);
+                    if (wait_ms) wait_ms[i] = r.start_ms - (base + arrival_ms[i]);
                     last = std::max(last, r.done_ms - base);
                     ppa_server_release(s, id);
                     ++done;

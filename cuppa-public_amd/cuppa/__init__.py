@@ -48,7 +48,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [("block_threads", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
                 ("chunk_nodes", ctypes.c_uint32), ("max_blocks", ctypes.c_uint32),
                 ("watchdog_ms", ctypes.c_uint32), ("early_nodes", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 2)]
+                ("sleep_cap", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
@@ -122,7 +122,7 @@ _SIGNATURES = [
     ("ppa_server_usage", ctypes.c_int, [_VP] + [ctypes.POINTER(ctypes.c_uint64)] * 4),
     ("ppa_server_replay", ctypes.c_int, [_VP, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double),
                                          ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
-                                         ctypes.POINTER(ctypes.c_double)]),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
 ]
 
 
@@ -133,6 +133,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"native library missing: {LIB_PATH} (run __graft_entry__.build())")
+    # torch ships its own libamdhip64.so.7. Loaded after ours it would be a
+    # second HIP runtime in the process that finds no GPU; loaded first, the
+    # library binds to it by soname and both share one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     for name, res, args in _SIGNATURES:
         f = getattr(L, name)
@@ -216,8 +223,8 @@ class Context:
         return self._h
 
     def set_tuning(self, block_threads=0, blocks_per_cu=0, chunk_nodes=0, max_blocks=0, watchdog_ms=0,
-                   early_nodes=0):
-        t = Tuning(block_threads, blocks_per_cu, chunk_nodes, max_blocks, watchdog_ms, early_nodes)
+                   early_nodes=0, sleep_cap=0):
+        t = Tuning(block_threads, blocks_per_cu, chunk_nodes, max_blocks, watchdog_ms, early_nodes, sleep_cap)
         _check(lib().ppa_ctx_set_tuning(self._h, ctypes.byref(t)), "ppa_ctx_set_tuning")
 
     def get_tuning(self) -> dict:
@@ -333,7 +340,7 @@ def autotune(ctx: "Context", descs, targets, row_ptr=None, stream=None) -> dict:
 class Server:
     """Continuous-batching request loop (ppa_server)."""
 
-    def __init__(self, ctx: "Context", budget_bytes: int, streams: int = 4, max_batch: int = 0,
+    def __init__(self, ctx: "Context", budget_bytes: int, streams: int = 0, max_batch: int = 0,
                  batch_nodes: int = 0, solo_nodes: int = 0, plain: bool = False):
         self.ctx = ctx
         self._h = ctypes.c_void_p()
@@ -365,14 +372,19 @@ class Server:
         _check(lib().ppa_server_usage(self._h, *[ctypes.byref(x) for x in v]), "ppa_server_usage")
         return dict(zip(("used", "budget", "queued", "running"), (x.value for x in v)))
 
-    def replay(self, specs, arrival_ms):
-        """specs: list of (id, n, d, p_threshold, seed). Returns (latency_ms list, makespan_ms)."""
+    def replay(self, specs, arrival_ms, with_wait=False):
+        """specs: list of (id, n, d, p_threshold, seed). Returns (latency_ms list,
+        makespan_ms), or (latency_ms, wait_ms, makespan_ms) with with_wait."""
         count = len(specs)
         reqs = (Request * count)(*[Request(i, n, d, 0, T, s) for (i, n, d, T, s) in specs])
         arr = (ctypes.c_double * count)(*arrival_ms)
         lat = (ctypes.c_double * count)()
+        wait = (ctypes.c_double * count)()
         mk = ctypes.c_double()
-        _check(lib().ppa_server_replay(self._h, reqs, arr, count, lat, ctypes.byref(mk)), "ppa_server_replay")
+        _check(lib().ppa_server_replay(self._h, reqs, arr, count, lat, wait, ctypes.byref(mk)),
+               "ppa_server_replay")
+        if with_wait:
+            return list(lat), list(wait), mk.value
         return list(lat), mk.value
 
     def close(self):

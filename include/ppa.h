@@ -80,7 +80,10 @@ typedef struct ppa_tuning {
     uint32_t early_nodes;    /* nodes below this index publish each slot as it is accepted and
                                 are claimed one per lane (0 = default 2^22, 1 = none); batches
                                 apply at most 16 chunks of it per graph */
-    uint32_t reserved[2];
+    uint32_t sleep_cap;      /* longest back-off of a wavefront whose lanes all wait, in units of
+                                256 clocks, 1..64 (0 = by grid size: 1 for small grids, 32 for a
+                                full GPU) */
+    uint32_t reserved;
 } ppa_tuning;
 
 /* Counters of the last generate call (cuPPA_1.cu:84-87 timing CSV and the
@@ -183,7 +186,8 @@ typedef struct ppa_server ppa_server;
 
 typedef struct ppa_server_config {
     uint64_t budget_bytes; /* device pool for request outputs */
-    uint32_t streams;      /* concurrent launches (1..8, 0 = 4) */
+    uint32_t streams;      /* concurrent launches (1..16, 0 = 16); they overlap only as far as the
+                              HIP runtime has hardware queues (GPU_MAX_HW_QUEUES, default 4) */
     uint32_t max_batch;    /* requests per launch (0 = 4096) */
     uint64_t batch_nodes;  /* a launch closes once it holds this many nodes (0 = 2^22) */
     uint64_t solo_nodes;   /* requests of at least this many nodes launch alone (0 = 2^20) */
@@ -231,9 +235,10 @@ int ppa_server_usage(ppa_server *srv, uint64_t *used_bytes, uint64_t *budget_byt
                      uint64_t *running);
 /* Replays a trace in real time: request i is submitted when the clock reaches
  * arrival_ms[i] (ms from the start of the call) and released as soon as it
- * finishes; latency_ms[i] = finish - arrival. makespan_ms = last finish. */
+ * finishes; latency_ms[i] = finish - arrival, wait_ms[i] (nullable) = launch
+ * start - arrival. makespan_ms = last finish. */
 int ppa_server_replay(ppa_server *srv, const ppa_request *reqs, const double *arrival_ms, uint64_t count,
-                      double *latency_ms, double *makespan_ms);
+                      double *latency_ms, double *wait_ms, double *makespan_ms);
 
 /* ---- serial CPU path (`-A scm`): replaces SerialCopyModel::generateGraph
  *      (src/PA/SPA/SerialCopyModel3.cpp:11-107) ---------------------------- */
