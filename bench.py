@@ -30,9 +30,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E vendor figure
 
 WORKLOADS = {
     # name: (n, d, p, seed)
-    "cfg3": (1 << 30, 8, 0.2, 11),
-    "cfg2": (1 << 28, 8, 0.5, 7),
-    "cfg1": (1_000_000, 4, 0.5, 42),
+    "n2p30_d8_p0.2": (1 << 30, 8, 0.2, 11),
+    "n2p28_d8_p0.5": (1 << 28, 8, 0.5, 7),
+    "n1e6_d4_p0.5": (1_000_000, 4, 0.5, 42),
 }
 
 
@@ -219,7 +219,7 @@ def run_serve(args):
     plain = replay(True, trace[:np_plain], plain_rate)
     # served: increasing arrival rates; each run covers at most ~10 s of arrivals
     served = []
-    for mult in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 16, 20, 24, 32):
+    for mult in [float(m) for m in args.serve_mults.split(",")]:
         rate = plain_rate * mult
         count = int(min(len(trace), max(np_plain, rate * 10)))
         res = replay(False, trace[:count], rate)
@@ -247,7 +247,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="n2p30_d8_p0.2", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-prefix", type=int, default=1 << 24,
                     help="nodes of the CPU baseline sample (0 = skip)")
     ap.add_argument("--degree", action="store_true", help="also time the degree-histogram pass")
@@ -260,6 +260,9 @@ def main():
     ap.add_argument("--serve-count", type=int, default=100_000)
     ap.add_argument("--serve-plain-count", type=int, default=2000)
     ap.add_argument("--serve-budget-gb", type=int, default=200)
+    ap.add_argument("--serve-mults", default="1,2,3,4,5,6,7,8,9,10,12,14,16,20,24,32",
+                    help="served arrival rates, as multiples of the plain rate, tried in order until the "
+                         "served p99 exceeds the plain p99")
     ap.add_argument("--serve-streams", type=int, default=16,
                     help="concurrent launches of the served loop (HIP hardware queues are raised to match)")
     args = ap.parse_args()

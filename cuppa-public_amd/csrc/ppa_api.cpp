@@ -55,6 +55,9 @@ int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *ou
     // memory with re-reads, so its all-waiting wavefronts sleep longer.
     const uint64_t waves = g * (th / 64);
     out->sleep_cap = tu.sleep_cap ? tu.sleep_cap : (uint32_t)std::min<uint64_t>(32, std::max<uint64_t>(1, waves / 256));
+    // Likewise a lane re-reads the empty sources behind its current slot every
+    // 4th pass in small grids and every 16th in large ones.
+    out->wide_mask = waves >= 4096 ? 15u : 3u;
     return PPA_OK;
 }
 
@@ -224,6 +227,7 @@ int ppa_generate_async(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targ
         ga.early_nodes = shape.early_nodes;
         ga.watchdog_ticks = shape.watchdog_ticks;
         ga.sleep_cap = shape.sleep_cap;
+        ga.wide_mask = shape.wide_mask;
         ga.scratch = ctx->scratch;
         PPA_HIP(launch_generate(d, ga, shape.blocks, shape.threads, stream));
     }

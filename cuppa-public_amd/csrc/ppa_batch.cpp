@@ -149,6 +149,7 @@ hipError_t enqueue_batch(void *d_tables, const BatchTables &t, const LaunchShape
     ga.early_nodes = std::min<uint64_t>(shape.early_nodes, 16ull * shape.chunk);
     ga.watchdog_ticks = shape.watchdog_ticks;
     ga.sleep_cap = shape.sleep_cap;
+    ga.wide_mask = shape.wide_mask;
     ga.scratch = scratch;
     return launch_generate_batch(ga, shape.blocks, shape.threads, stream);
 }

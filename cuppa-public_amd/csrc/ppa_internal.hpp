@@ -61,6 +61,7 @@ struct LaunchShape {
     uint64_t early_nodes = 0;
     uint64_t watchdog_ticks = 0;
     uint32_t sleep_cap = 1;
+    uint32_t wide_mask = 3;
 };
 
 // Grid, workgroup and claim sizes for `work` generated nodes with out-degree

@@ -28,7 +28,7 @@
 //     rejected attempt (duplicate) or an empty source slot costs another
 //     round trip. An empty source is kept in its slot as -2 - source, and a
 //     node's empty sources are re-read together (the first one every pass,
-//     the others every fourth), so a node waits for its slowest source, not
+//     the others every 4th or, in large grids, 16th), so a node waits for its slowest source, not
 //     for the sum of its sources' waits. The reference instead parks
 //     placeholders -(v*M+k) in a per-thread LDS queue and spins
 //     (cuPPA_RRP_1.cu:97-185).
@@ -403,7 +403,7 @@ void generate_kernel(GenArgs a)
             // Coherent re-reads, one round trip per pass: the source of a
             // redrawn slot k (row 0 of the LDS info area), and the attempt-0
             // sources still empty -- slot k every pass (batches: with a
-            // back-off), the later ones every fourth pass -- in row kk of the
+            // back-off), the later ones every 4th or 16th pass -- in row kk of the
             // LDS stage. Acceptance stays in slot order, but the waits of a
             // node's slots overlap. Agent-scope global_load_lds: no register
             // holds a read in flight.
@@ -415,7 +415,7 @@ void generate_kernel(GenArgs a)
             } else if (have_src) {
                 lpoll = true;
             }
-            const bool wide = (iter & 3u) == 0u;
+            const bool wide = (iter & a.wide_mask) == 0u;
             const bool head_ok = !(BATCH && skip > 0);
             uint32_t pmask = 0;
 #pragma unroll

@@ -69,7 +69,7 @@ struct GenArgs {
     uint64_t watchdog_ticks;  // wall-clock ticks a wavefront may run
     DeviceScratch *scratch;
     uint32_t sleep_cap;       // longest wavefront back-off, in 256-clock sleeps (>= 1)
-    uint32_t pad;
+    uint32_t wide_mask;       // later pending slots are re-read when (pass & wide_mask) == 0
 };
 
 struct FillArgs {

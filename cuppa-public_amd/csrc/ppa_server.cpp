@@ -201,6 +201,7 @@ int launch(ppa_server *s, Slot &sl, const std::vector<uint64_t> &ids)
             ga.early_nodes = shape.early_nodes;
             ga.watchdog_ticks = shape.watchdog_ticks;
             ga.sleep_cap = shape.sleep_cap;
+            ga.wide_mask = shape.wide_mask;
             ga.scratch = sl.scratch;
             PPA_HIP(launch_generate(r.req.d, ga, shape.blocks, shape.threads, sl.stream));
         }

@@ -2,7 +2,7 @@
 
 * tiny_graphs.json -- complete target arrays of small graphs from the
   pure-Python restatement in oracle/oracle.py (independent of the C oracle).
-* cfg1_digest.json -- SHA-256 of the n=1e6, d=4, p=0.5, seed=42 target array
+* n1e6_d4_p0.5_digest.json -- SHA-256 of the n=1e6, d=4, p=0.5, seed=42 target array
   from the C oracle, plus sampled row slices.
 * ref_dd_n<N>_d<D>_p<P>.csv -- "Degree,Count" distributions written by the
   reference's own serial copy model (src/PA/SPA/SerialCopyModel3.cpp built by
@@ -40,7 +40,7 @@ def main():
     n, d, p, seed = 1_000_000, 4, 0.5, 42
     G, _, retries = oracle.generate(n, d, oracle.p_threshold(p), seed)
     slices = {str(s): G[s * d:(s + 16) * d].tolist() for s in (5, 1000, 123_457, 500_000, n - 16)}
-    with open(os.path.join(HERE, "cfg1_digest.json"), "w") as f:
+    with open(os.path.join(HERE, "n1e6_d4_p0.5_digest.json"), "w") as f:
         json.dump({"n": n, "d": d, "p": p, "seed": seed, "retries": retries,
                    "sha256": hashlib.sha256(G.tobytes()).hexdigest(), "slices": slices}, f)
 

@@ -17,8 +17,8 @@ def test_tiny_graphs_fixture(oracle_mod):
         assert G.tolist() == g["targets"], g
 
 
-def test_cfg1_digest(oracle_mod):
-    dg = json.load(open(os.path.join(GOLDEN, "cfg1_digest.json")))
+def test_n1e6_d4_digest(oracle_mod):
+    dg = json.load(open(os.path.join(GOLDEN, "n1e6_d4_p0.5_digest.json")))
     n, d = dg["n"], dg["d"]
     G, _, retries = oracle_mod.generate(n, d, oracle_mod.p_threshold(dg["p"]), dg["seed"])
     assert hashlib.sha256(G.tobytes()).hexdigest() == dg["sha256"]

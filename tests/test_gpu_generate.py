@@ -37,7 +37,7 @@ def test_small_graphs_bit_exact(ctx, oracle_mod, n, d, p, seed):
     assert st["nodes"] == n - d - 1
 
 
-def test_cfg1_bit_exact(ctx, oracle_mod):
+def test_n1e6_d4_bit_exact(ctx, oracle_mod):
     n, d, p, seed = 1_000_000, 4, 0.5, 42
     T = oracle_mod.p_threshold(p)
     want, _, _ = oracle_mod.generate(n, d, T, seed)

@@ -18,8 +18,8 @@ def test_tiny_graphs_fixture(ctx):
         assert out.targets.cpu().tolist() == g["targets"], g
 
 
-def test_cfg1_digest(ctx):
-    dg = json.load(open(os.path.join(GOLDEN, "cfg1_digest.json")))
+def test_n1e6_d4_digest(ctx):
+    dg = json.load(open(os.path.join(GOLDEN, "n1e6_d4_p0.5_digest.json")))
     g = ctx.generate(dg["n"], dg["d"], dg["p"], dg["seed"])
     G = g.targets.cpu().numpy()
     assert hashlib.sha256(G.tobytes()).hexdigest() == dg["sha256"]

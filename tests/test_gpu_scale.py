@@ -114,7 +114,7 @@ def test_full_degree_histogram(ctx, big_graph):
     assert int((h * x).sum()) == d * (d + 1) + 2 * (n - d - 1) * d
 
 
-def test_cfg3_sampled_rows_consistent(ctx, big_graph, oracle_mod):
+def test_n2p30_sampled_rows_consistent(ctx, big_graph, oracle_mod):
     import torch
     n, d, p, seed = 1 << 30, 8, 0.2, 11
     T = oracle_mod.p_threshold(p)
