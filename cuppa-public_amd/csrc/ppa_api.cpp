@@ -10,6 +10,7 @@
 #include <cstring>
 #include <new>
 
+#include "ppa_degree.hpp"
 #include "ppa_internal.hpp"
 
 namespace ppa {
@@ -167,6 +168,7 @@ int ppa_ctx_destroy(ppa_ctx *ctx)
         if (e) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->degree_scratch) (void)hipFree(ctx->degree_scratch);
+    if (ctx->degree_work) (void)hipFree(ctx->degree_work);
     delete ctx;
     return PPA_OK;
 }
@@ -277,7 +279,26 @@ int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t
         deg = ctx->degree_scratch;
     }
     PPA_HIP(launch_reset_scratch(ctx->scratch, 0, stream));
-    PPA_HIP(launch_degree(d_targets, desc->n, desc->d, deg, d_hist, hist_len, ctx->scratch, stream));
+    // Partitioned pass (two int32 copies of the slots plus small tables, kept
+    // for the next call); one atomic per slot only if they cannot be had.
+    const size_t need = degree_work_bytes(desc->n, desc->d, (uint32_t)ctx->num_cus);
+    if (ctx->degree_work_bytes < need) {
+        PPA_HIP(hipStreamSynchronize(stream));
+        if (ctx->degree_work) PPA_HIP(hipFree(ctx->degree_work));
+        ctx->degree_work = nullptr;
+        ctx->degree_work_bytes = 0;
+        if (hipMalloc(&ctx->degree_work, need) == hipSuccess) ctx->degree_work_bytes = need;
+        else ctx->degree_work = nullptr;
+        (void)hipGetLastError();
+    }
+    if (ctx->degree_work) {
+        DegreeWork w;
+        degree_work_layout(ctx->degree_work, desc->n, desc->d, (uint32_t)ctx->num_cus, &w);
+        PPA_HIP(launch_degree_partitioned(d_targets, desc->n, desc->d, deg, w, stream));
+    } else {
+        PPA_HIP(launch_degree_atomic(d_targets, desc->n, desc->d, deg, stream));
+    }
+    PPA_HIP(launch_degree_histogram(deg, desc->n, d_hist, hist_len, ctx->scratch, stream));
     if (d_max_degree)
         PPA_HIP(hipMemcpyAsync(d_max_degree, &ctx->scratch->max_degree, sizeof(uint64_t),
                                hipMemcpyDeviceToDevice, stream));

@@ -1,0 +1,34 @@
+// ppa_degree.hpp -- the partitioned degree pass (ppa_degree.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ppa {
+
+// Device buffers of one partitioned degree pass, carved from one allocation
+// by degree_work_layout.
+struct DegreeWork {
+    int32_t *part_a = nullptr;      // slots after the first partition
+    int32_t *part_b = nullptr;      // slots after the second partition (two-level graphs)
+    uint64_t *counts = nullptr;     // per (segment, bin, workgroup) counts
+    uint64_t *offsets = nullptr;    // their exclusive scan, one extra entry for the total
+    uint64_t *partials = nullptr;   // scan workgroup sums
+    uint32_t *seg_b0 = nullptr;     // 2 levels x (segments + 1): first workgroup per segment
+    uint64_t *seg_start = nullptr;  // 2 levels x (segments + 1): first slot per segment
+    uint64_t *leaf_size = nullptr;  // slots per leaf of 2^14 nodes
+    uint64_t *leaf_start = nullptr; // their exclusive scan
+    uint64_t *piece_base = nullptr; // exclusive scan of pieces per leaf
+    uint32_t num_cus = 0;           // compute units the launch shapes are sized for
+};
+
+// Bytes of the DegreeWork of an n-node, out-degree-d graph on a GPU with
+// num_cus compute units.
+size_t degree_work_bytes(uint64_t n, uint32_t d, uint32_t num_cus);
+void degree_work_layout(void *base, uint64_t n, uint32_t d, uint32_t num_cus, DegreeWork *w);
+
+// deg[v] = d + #{slots of nodes t >= d+1 naming v} for v < n.
+hipError_t launch_degree_partitioned(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, const DegreeWork &w,
+                                     hipStream_t stream);
+
+}  // namespace ppa

@@ -18,6 +18,8 @@ struct ppa_ctx {
     ppa::DeviceScratch *scratch = nullptr;  // device
     uint32_t *degree_scratch = nullptr;     // device, grown on demand
     uint64_t degree_scratch_len = 0;
+    void *degree_work = nullptr;            // device buffers of the partitioned degree pass
+    size_t degree_work_bytes = 0;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     ppa_stats last{};
     bool pending = false;        // a launch awaits ppa_ctx_sync / ppa_batch_sync

@@ -646,8 +646,10 @@ __global__ void degree_init_kernel(uint32_t *deg, uint64_t n, uint32_t d)
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) deg[v] = d;
 }
 
-// deg[target]++ for every slot of nodes t >= d+1 (cuPPA.cu:115-131).
-// Out-of-range values (an incomplete graph) are skipped, never dereferenced.
+// deg[target]++ for every slot of nodes t >= d+1 (cuPPA.cu:115-131), one
+// atomic per slot: the fallback when the partitioned pass (ppa_degree.hip)
+// cannot get its buffers. Out-of-range values (an incomplete graph) are
+// skipped, never dereferenced.
 __global__ void degree_count_kernel(const int32_t *G, uint64_t first_slot, uint64_t slots, uint64_t n,
                                     uint32_t *deg)
 {
@@ -775,14 +777,19 @@ hipError_t generate_occupancy(uint32_t d, uint32_t threads, int *out)
     }
 }
 
-hipError_t launch_degree(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, uint64_t *hist,
-                         uint64_t hist_len, DeviceScratch *s, hipStream_t stream)
+hipError_t launch_degree_atomic(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, hipStream_t stream)
 {
     hipLaunchKernelGGL(degree_init_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, stream, deg, n, d);
     const uint64_t first = (uint64_t)(d + 1) * d, slots = n * d;
     if (slots > first)
         hipLaunchKernelGGL(degree_count_kernel, dim3(grid_for(slots - first, 256, 16384)), dim3(256), 0,
                            stream, G, first, slots, n, deg);
+    return hipGetLastError();
+}
+
+hipError_t launch_degree_histogram(const uint32_t *deg, uint64_t n, uint64_t *hist, uint64_t hist_len,
+                                   DeviceScratch *s, hipStream_t stream)
+{
     if (hist && hist_len)
         hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_for(hist_len, 256, 4096)), dim3(256), 0, stream, hist,
                            hist_len);

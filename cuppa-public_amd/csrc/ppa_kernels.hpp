@@ -95,8 +95,9 @@ hipError_t launch_generate(uint32_t d, const GenArgs &a, uint32_t blocks, uint32
                            hipStream_t stream);
 hipError_t launch_generate_batch(const GenArgs &a, uint32_t blocks, uint32_t block_threads, hipStream_t stream);
 hipError_t generate_occupancy(uint32_t d, uint32_t block_threads, int *blocks_per_cu);
-hipError_t launch_degree(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, uint64_t *hist,
-                         uint64_t hist_len, DeviceScratch *s, hipStream_t stream);
+hipError_t launch_degree_atomic(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, hipStream_t stream);
+hipError_t launch_degree_histogram(const uint32_t *deg, uint64_t n, uint64_t *hist, uint64_t hist_len,
+                                   DeviceScratch *s, hipStream_t stream);
 
 constexpr uint32_t kBatchDMax = 16;  // d of 0 selects the batch kernel in generate_occupancy
 

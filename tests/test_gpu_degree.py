@@ -1,0 +1,50 @@
+"""Degree pass on the GPU against numpy counting, across the pass's regimes:
+no partition (n <= 2^14), one partition level (n <= 2^22), two levels, and
+leaves of more than 2^20 slots that are counted in pieces. Target arrays are
+synthetic (heavy-tailed towards node 0, like a preferential-attachment graph,
+with a few out-of-range values that must be skipped), so the cases cost no
+generation time; generated graphs are covered in test_gpu_generate and
+test_gpu_scale."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _targets(n, d, seed, power):
+    rng = np.random.default_rng(seed)
+    G = np.empty(n * d, np.int32)
+    for i in range(d + 1):
+        G[i * d:(i + 1) * d] = [j + (j >= i) for j in range(d)]
+    t = np.arange(d + 1, n, dtype=np.int64)
+    G[(d + 1) * d:] = (t[:, None] * rng.random((len(t), d)) ** power).astype(np.int32).ravel()
+    if n * d > (d + 1) * d + 8:
+        bad = rng.integers((d + 1) * d, n * d, 6)
+        G[bad[:3]] = -1
+        G[bad[3:]] = n + 5
+    return G
+
+
+@pytest.mark.parametrize("n,d,power", [
+    (300, 4, 3),                 # one leaf, no partition
+    (16384, 2, 3),               # exactly one leaf
+    (20000, 3, 3),               # one partition level, two leaves
+    (1 << 20, 2, 3),
+    (3 << 20, 5, 3),
+    ((1 << 22) + 77777, 2, 3),   # two levels, two level-1 buckets
+    (9_000_000, 1, 3),
+    ((1 << 24) + 5, 8, 4),       # two levels, hub leaves split into pieces
+])
+def test_degrees_match_counting(ctx, n, d, power):
+    import torch
+    import cuppa
+    G = _targets(n, d, 7, power)
+    g = cuppa.Graph(n, d, torch.from_numpy(G).cuda(), None)
+    hist, mx, deg = ctx.degree_histogram(g, want_degree=True)
+    v = G[(d + 1) * d:]
+    v = v[(v >= 0) & (v < n)]
+    want = d + np.bincount(v, minlength=n)
+    got = deg.cpu().numpy().astype(np.int64)
+    assert np.array_equal(got, want)
+    assert mx == int(want.max())
+    assert np.array_equal(hist.cpu().numpy(), np.bincount(want))
