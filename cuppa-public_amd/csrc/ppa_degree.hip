@@ -215,9 +215,17 @@ __global__ __launch_bounds__(kThreads) void part_count_kernel(const int32_t *in,
     const BlockRange r = block_range(pl, blockIdx.x);
     for (uint32_t i = threadIdx.x; i < pl.nbins; i += kThreads) hist[i] = 0;
     __syncthreads();
-    for (uint64_t i = r.lo + threadIdx.x; i < r.hi; i += kThreads) {
-        const int32_t v = in[i];
-        if (valid(v, n)) atomicAdd(&hist[((uint32_t)v >> pl.shift) & pl.mask], 1u);
+    // kPerThread reads in flight per thread, then their LDS updates
+    for (uint64_t t0 = r.lo; t0 < r.hi; t0 += kTile) {
+        int32_t v[kPerThread];
+#pragma unroll
+        for (int j = 0; j < kPerThread; ++j) {
+            const uint64_t i = t0 + (uint64_t)j * kThreads + threadIdx.x;
+            v[j] = i < r.hi ? in[i] : -1;
+        }
+#pragma unroll
+        for (int j = 0; j < kPerThread; ++j)
+            if (valid(v[j], n)) atomicAdd(&hist[((uint32_t)v[j] >> pl.shift) & pl.mask], 1u);
     }
     __syncthreads();
     for (uint32_t bin = threadIdx.x; bin < pl.nbins; bin += kThreads) {
@@ -380,9 +388,16 @@ __global__ __launch_bounds__(kThreads) void leaf_count_kernel(const int32_t *in,
         const uint64_t end = first + kPiece < leaf_start[lo + 1] ? first + kPiece : leaf_start[lo + 1];
         for (uint32_t i = threadIdx.x; i < kLeafNodes; i += kThreads) cnt[i] = 0;
         __syncthreads();
-        for (uint64_t i = first + threadIdx.x; i < end; i += kThreads) {
-            const int32_t v = in[i];
-            if (valid(v, n)) atomicAdd(&cnt[(uint32_t)v & (kLeafNodes - 1)], 1u);
+        for (uint64_t t0 = first; t0 < end; t0 += kTile) {
+            int32_t v[kPerThread];
+#pragma unroll
+            for (int j = 0; j < kPerThread; ++j) {
+                const uint64_t i = t0 + (uint64_t)j * kThreads + threadIdx.x;
+                v[j] = i < end ? in[i] : -1;
+            }
+#pragma unroll
+            for (int j = 0; j < kPerThread; ++j)
+                if (valid(v[j], n)) atomicAdd(&cnt[(uint32_t)v[j] & (kLeafNodes - 1)], 1u);
         }
         __syncthreads();
         const uint64_t base = (uint64_t)lo << kLeafBits;
