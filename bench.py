@@ -57,6 +57,45 @@ def read_traffic(profile_dir: str, workload: str):
     return None
 
 
+def host_info():
+    """The host the CPU baseline ran on."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            info["cpu_model"] = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    try:
+        with open("/proc/meminfo") as f:
+            kb = next(int(l.split()[1]) for l in f if l.startswith("MemTotal"))
+        info["ram_gb"] = round(kb / 2**20, 1)
+    except (OSError, StopIteration):
+        pass
+    return info
+
+
+def _run_ref(n: int, d: int, p: float):
+    """One run of the reference serial copy model on core 0; its own timer."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "scm_ref")
+    with tempfile.TemporaryDirectory() as tmp:
+        cmd = [ref, str(n), str(d), repr(p), tmp + "/", "0"]
+        cmd = ["taskset", "-c", "0"] + cmd if _has("taskset") else cmd
+        out = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=300).stdout
+        return float(re.search(r"Computation Time\s*:\s*([0-9.eE+-]+)", out).group(1))
+
+
+def cpu_mix_baseline(graphs, what: str):
+    """Reference serial copy model (one core) over a sample of a graph mix:
+    graphs = [(n, d, p)]; edges/s = sample edges / summed run time."""
+    if not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "scm_ref")):
+        return None
+    secs = sum(_run_ref(n, d, p) for n, d, p in graphs)
+    edges = sum(n * d for n, d, _ in graphs)
+    return {"value": edges / secs, "unit": "edges/s", "cores": 1, "kind": "reference",
+            "sample": f"reference SerialCopyModel (MT19937-64) on {len(graphs)} graphs of the {what} "
+                      f"({edges:.3g} edges)", "seconds": secs, "host": host_info()}
+
+
 def cpu_baseline(n: int, d: int, p: float, prefix: int):
     """Reference serial copy model (one core) over the first `prefix` nodes."""
     ref = os.path.join(ROOT, "oracle", "_ref", "scm_ref")
@@ -69,7 +108,7 @@ def cpu_baseline(n: int, d: int, p: float, prefix: int):
                 out = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=300).stdout
                 secs = float(re.search(r"Computation Time\s*:\s*([0-9.eE+-]+)", out).group(1))
                 return {"value": prefix * d / secs, "unit": "edges/s", "cores": 1, "kind": "reference",
-                        "sample": sample, "seconds": secs}
+                        "sample": sample, "seconds": secs, "host": host_info()}
             except Exception as e:  # fall through to the port
                 sample += f" (reference run failed: {e})"
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -79,7 +118,8 @@ def cpu_baseline(n: int, d: int, p: float, prefix: int):
     oracle.generate(prefix, d, T, 11)
     secs = time.perf_counter() - t0
     return {"value": prefix * d / secs, "unit": "edges/s", "cores": 1, "kind": "port",
-            "sample": f"oracle restatement, first {prefix} nodes of n={n}, d={d}, p={p}", "seconds": secs}
+            "sample": f"oracle restatement, first {prefix} nodes of n={n}, d={d}, p={p}", "seconds": secs,
+            "host": host_info()}
 
 
 def _has(tool: str) -> bool:
@@ -156,6 +196,9 @@ def run_batch(args):
         ctx.generate_into(cuppa.GraphDesc(n, d, 0, T, seed, 0, 0), scratch, scratch_rp)
     torch.cuda.synchronize()
     plain = time.perf_counter() - t0
+    # the reference's serial path on every 256th graph of the ensemble (~10 s of one core)
+    cpu = cpu_mix_baseline([(n, d, T / 2**32) for n, d, T, _ in specs[::256]], "ensemble") \
+        if args.cpu_prefix else None
     print(json.dumps({
         "metric": "batched ensemble edges/s (one hipGraph launch) vs plain per-graph launches",
         "value": edges / batched, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
@@ -163,7 +206,7 @@ def run_batch(args):
         "config": {"workload": f"{len(specs)} copy-model graphs, n log-uniform [1e3,1e7], d 1..16, "
                                f"p [0.05,0.95], seed 1000+i", "graphs": len(specs), "edges": edges},
         "batched_ms": batched * 1e3, "plain_ms": plain * 1e3, "plain_edges_per_s": edges / plain,
-        "speedup_vs_plain": plain / batched, "tuning": tuned,
+        "speedup_vs_plain": plain / batched, "tuning": tuned, "cpu_baseline": cpu,
         "stats": {k: st[k] for k in ("nodes", "draws", "retries", "prefetch_misses", "polls", "staged_rows",
                                      "grid_blocks", "block_threads")},
     }))
@@ -239,6 +282,9 @@ def run_serve(args):
                    "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "plain_saturation": sat, "plain": plain, "served": served,
         "throughput_ratio_at_equal_p99": (best["edges_per_s"] / plain["edges_per_s"]) if best else None,
+        # the reference's serial path on the first 64 requests of the trace (one core)
+        "cpu_baseline": cpu_mix_baseline([(n, d, T / 2**32) for _, n, d, T, _ in trace[:64]], "request mix")
+        if args.cpu_prefix else None,
     }))
 
 
@@ -248,7 +294,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="n2p30_d8_p0.2", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-prefix", type=int, default=1 << 24,
+    ap.add_argument("--cpu-prefix", type=int, default=1 << 25,
                     help="nodes of the CPU baseline sample (0 = skip)")
     ap.add_argument("--degree", action="store_true", help="also time the degree-histogram pass")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
