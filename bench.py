@@ -297,6 +297,7 @@ def main():
     ap.add_argument("--cpu-prefix", type=int, default=1 << 25,
                     help="nodes of the CPU baseline sample (0 = skip)")
     ap.add_argument("--degree", action="store_true", help="also time the degree-histogram pass")
+    ap.add_argument("--in-csr", action="store_true", help="also time the in-edge CSR build")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB three times (counter calibration) and exit")
     ap.add_argument("--mode", default="graph", choices=("graph", "batch", "serve"),
@@ -391,6 +392,20 @@ def main():
         extra["degree_pass_algorithmic_gbs"] = alg / (times[1] * 1e-3) / 1e9
         extra["degree_pass_roofline_frac"] = alg / (times[1] * 1e-3) / 1e9 / HBM_PEAK_GBS
         extra["max_degree"] = mx
+    if args.in_csr:
+        # the first call also allocates the work area; the second is timed
+        g = cuppa.Graph(n, d, targets, row_ptr)
+        times = []
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            in_ptr, in_src = ctx.in_csr(g)
+            torch.cuda.synchronize()
+            times.append((time.perf_counter() - t1) * 1e3)
+            del in_ptr, in_src
+        extra["in_csr_ms"] = times[1]
+        extra["in_csr_first_call_ms"] = times[0]
+        extra["in_csr_edges_per_s"] = edges / (times[1] * 1e-3)
     copy_gbs = copy_bandwidth_gbs(torch)
 
     del targets, row_ptr

@@ -62,6 +62,22 @@ int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *ou
     return PPA_OK;
 }
 
+int ensure_work(ppa_ctx *ctx, size_t need, hipStream_t stream)
+{
+    if (ctx->work_bytes >= need) return PPA_OK;
+    PPA_HIP(hipStreamSynchronize(stream));
+    if (ctx->work) PPA_HIP(hipFree(ctx->work));
+    ctx->work = nullptr;
+    ctx->work_bytes = 0;
+    if (hipMalloc(&ctx->work, need) != hipSuccess) {
+        ctx->work = nullptr;
+        (void)hipGetLastError();
+        return PPA_ERR_NOMEM;
+    }
+    ctx->work_bytes = need;
+    return PPA_OK;
+}
+
 int finish_launch(ppa_ctx *ctx, hipStream_t stream)
 {
     PPA_HIP(hipStreamSynchronize(stream));
@@ -168,7 +184,7 @@ int ppa_ctx_destroy(ppa_ctx *ctx)
         if (e) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->degree_scratch) (void)hipFree(ctx->degree_scratch);
-    if (ctx->degree_work) (void)hipFree(ctx->degree_work);
+    if (ctx->work) (void)hipFree(ctx->work);
     delete ctx;
     return PPA_OK;
 }
@@ -281,19 +297,9 @@ int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t
     PPA_HIP(launch_reset_scratch(ctx->scratch, 0, stream));
     // Partitioned pass (two int32 copies of the slots plus small tables, kept
     // for the next call); one atomic per slot only if they cannot be had.
-    const size_t need = degree_work_bytes(desc->n, desc->d, (uint32_t)ctx->num_cus);
-    if (ctx->degree_work_bytes < need) {
-        PPA_HIP(hipStreamSynchronize(stream));
-        if (ctx->degree_work) PPA_HIP(hipFree(ctx->degree_work));
-        ctx->degree_work = nullptr;
-        ctx->degree_work_bytes = 0;
-        if (hipMalloc(&ctx->degree_work, need) == hipSuccess) ctx->degree_work_bytes = need;
-        else ctx->degree_work = nullptr;
-        (void)hipGetLastError();
-    }
-    if (ctx->degree_work) {
+    if (ensure_work(ctx, degree_work_bytes(desc->n, desc->d, (uint32_t)ctx->num_cus), stream) == PPA_OK) {
         DegreeWork w;
-        degree_work_layout(ctx->degree_work, desc->n, desc->d, (uint32_t)ctx->num_cus, &w);
+        degree_work_layout(ctx->work, desc->n, desc->d, (uint32_t)ctx->num_cus, &w);
         PPA_HIP(launch_degree_partitioned(d_targets, desc->n, desc->d, deg, w, stream));
     } else {
         PPA_HIP(launch_degree_atomic(d_targets, desc->n, desc->d, deg, stream));
@@ -302,6 +308,23 @@ int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t
     if (d_max_degree)
         PPA_HIP(hipMemcpyAsync(d_max_degree, &ctx->scratch->max_degree, sizeof(uint64_t),
                                hipMemcpyDeviceToDevice, stream));
+    PPA_HIP(hipStreamSynchronize(stream));
+    return PPA_OK;
+}
+
+int ppa_in_csr(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_targets, int64_t *d_in_ptr,
+               int32_t *d_in_src, void *stream_)
+{
+    if (!ctx || !d_targets || !d_in_ptr || !d_in_src) return PPA_ERR_INVALID;
+    int rc = ppa_check_desc(desc);
+    if (rc) return rc;
+    DeviceGuard g(ctx->device);
+    if (!g.ok()) return PPA_ERR_DEVICE;
+    hipStream_t stream = (hipStream_t)stream_;
+    if ((rc = ensure_work(ctx, csr_work_bytes(desc->n, desc->d, (uint32_t)ctx->num_cus), stream)) != PPA_OK)
+        return rc;
+    PPA_HIP(launch_in_csr(d_targets, desc->n, desc->d, d_in_ptr, d_in_src, ctx->work, (uint32_t)ctx->num_cus,
+                          stream));
     PPA_HIP(hipStreamSynchronize(stream));
     return PPA_OK;
 }

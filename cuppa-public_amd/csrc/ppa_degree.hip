@@ -443,6 +443,152 @@ hipError_t partition(const int32_t *in, int32_t *out, const PartPlan &pl, uint32
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// ---- in-edge CSR: a stable LSD radix sort of the slots by target ------------
+// Pass p orders the (target, source) pairs by bits [8p, 8p+8) of the target,
+// keeping the order of equal digits; after ceil(bits(n-1) / 8) passes the
+// sources of each target are in slot order, which is ascending source order.
+// Ranks are exact without LDS atomics: a wavefront owns 1024 consecutive
+// slots of a 4096-slot tile and walks them 64 at a time; lanes with equal
+// digits are matched with 8 ballots and counted from a wavefront-private
+// running count, and the four wavefronts' counts are combined per digit.
+constexpr uint32_t kRadixBits = 8;
+constexpr uint32_t kRadixBins = 1u << kRadixBits;
+constexpr uint32_t kWaves = kThreads / 64;
+constexpr uint32_t kWaveShare = kTile / kWaves;   // slots per wavefront per tile
+constexpr uint32_t kSteps = kWaveShare / 64;      // 64-slot steps per wavefront
+
+// Lanes among `act` whose digit equals this lane's; whole wavefront.
+__device__ __forceinline__ uint64_t match_digit(uint32_t digit, bool act)
+{
+    uint64_t eq = __ballot(act);
+#pragma unroll
+    for (uint32_t b = 0; b < kRadixBits; ++b) {
+        const bool bit = (digit >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        eq &= bit ? m : ~m;
+    }
+    return eq;
+}
+
+// The number of valid pairs left by the first pass (offsets[len]) becomes
+// the single segment of every later pass.
+__global__ void plan_total_kernel(uint32_t *seg_b0, uint64_t *seg_start, uint32_t blocks, const uint64_t *total)
+{
+    if (threadIdx.x == 0) {
+        seg_b0[0] = 0;
+        seg_b0[1] = blocks;
+        seg_start[0] = 0;
+        seg_start[1] = *total;
+    }
+}
+
+// One stable pass. vin == nullptr: the sources are the slot indices' rows,
+// (first + i) / d (first pass, reading the targets array itself). kout may be
+// null (no later pass needs the keys).
+__global__ __launch_bounds__(kThreads) void lsd_scatter_kernel(const int32_t *kin, const int32_t *vin, int32_t *kout,
+                                                               int32_t *vout, PartPlan pl, uint64_t n, uint32_t d,
+                                                               double inv_d, uint64_t first, const uint64_t *offsets)
+{
+    __shared__ uint64_t cur[kRadixBins];            // next output position per digit
+    __shared__ uint32_t wrun[kWaves][kRadixBins];   // per-wavefront counts, then their prefix over wavefronts
+    __shared__ uint32_t tstart[kRadixBins];         // tile start of each digit
+    __shared__ int32_t tk[kTile], tv[kTile];
+    __shared__ uint64_t red[kWaves];
+    if (blockIdx.x >= pl.seg_b0[pl.nseg]) return;
+    const BlockRange r = block_range(pl, blockIdx.x);
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    cur[threadIdx.x] = offsets[r.count_base + (uint64_t)threadIdx.x * r.nb + r.rank];
+    __syncthreads();
+    for (uint64_t t0 = r.lo; t0 < r.hi; t0 += kTile) {
+        for (uint32_t b = lane; b < kRadixBins; b += 64) wrun[wave][b] = 0;
+        int32_t k[kSteps], v[kSteps];
+        uint32_t rk[kSteps];
+#pragma unroll
+        for (uint32_t j = 0; j < kSteps; ++j) {
+            const uint64_t i = t0 + wave * kWaveShare + j * 64 + lane;
+            k[j] = i < r.hi ? kin[i] : -1;
+            v[j] = vin ? (i < r.hi ? vin[i] : 0) : (int32_t)div_small(first + i, d, inv_d);
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kSteps; ++j) {
+            const bool act = valid(k[j], n);
+            const uint32_t dig = ((uint32_t)k[j] >> pl.shift) & (kRadixBins - 1);
+            const uint64_t eq = match_digit(dig, act);
+            const uint32_t below = rank_below(eq);
+            const uint32_t base = act ? wrun[wave][dig] : 0u;
+            __builtin_amdgcn_wave_barrier();
+            if (act && below == 0) wrun[wave][dig] = base + (uint32_t)__popcll(eq);
+            __builtin_amdgcn_wave_barrier();
+            rk[j] = base + below;
+        }
+        __syncthreads();
+        // thread = digit: prefix of the wavefronts' counts, and the tile total
+        uint32_t tot = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kWaves; ++w) {
+            const uint32_t c = wrun[w][threadIdx.x];
+            wrun[w][threadIdx.x] = tot;
+            tot += c;
+        }
+        uint64_t total = 0;
+        tstart[threadIdx.x] = (uint32_t)block_exclusive_scan<kThreads>(tot, &total, red);
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < kSteps; ++j) {
+            if (valid(k[j], n)) {
+                const uint32_t dig = ((uint32_t)k[j] >> pl.shift) & (kRadixBins - 1);
+                const uint32_t pos = tstart[dig] + wrun[wave][dig] + rk[j];
+                tk[pos] = k[j];
+                tv[pos] = v[j];
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < (uint32_t)total; i += kThreads) {
+            const int32_t kk = tk[i];
+            const uint32_t dig = ((uint32_t)kk >> pl.shift) & (kRadixBins - 1);
+            const uint64_t o = cur[dig] + (i - tstart[dig]);
+            if (kout) kout[o] = kk;
+            vout[o] = tv[i];
+        }
+        __syncthreads();
+        cur[threadIdx.x] += tot;
+        __syncthreads();
+    }
+}
+
+// in_ptr[v] = first position of target v in the sorted keys, for v in
+// (keys[i-1], keys[i]]; in_ptr[n] = total.
+__global__ void csr_ptr_kernel(const int32_t *keys, const uint64_t *total_p, uint64_t n, uint64_t cap,
+                               int64_t *in_ptr)
+{
+    const uint64_t total = *total_p;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= total && i <= cap; i += stride) {
+        const int64_t prev = i == 0 ? -1 : (int64_t)keys[i - 1];
+        const int64_t cur = i == total ? (int64_t)n : (int64_t)keys[i];
+        for (int64_t v = prev + 1; v <= cur; ++v) in_ptr[v] = (int64_t)i;
+    }
+}
+
+struct CsrGeometry {
+    uint64_t slots = 0;
+    uint32_t passes = 1, blocks = 1;
+    uint64_t counts = 0;
+};
+
+CsrGeometry csr_geometry(uint64_t n, uint32_t d, uint32_t num_cus)
+{
+    CsrGeometry g;
+    g.slots = n * (uint64_t)d;
+    uint32_t bits = 1;
+    while (bits < 63 && (1ull << bits) < n) ++bits;  // targets are < n <= 2^bits
+    g.passes = (bits + kRadixBits - 1) / kRadixBits;
+    uint64_t b = std::min<uint64_t>((uint64_t)num_cus * 4, kMaxBlocks);
+    g.blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, (g.slots + 65535) / 65536));
+    g.counts = (uint64_t)kRadixBins * g.blocks + 1;
+    return g;
+}
+
 }  // namespace
 
 size_t degree_work_bytes(uint64_t n, uint32_t d, uint32_t num_cus)
@@ -522,6 +668,58 @@ hipError_t launch_degree_partitioned(const int32_t *G, uint64_t n, uint32_t d, u
                        g.leaves, w.leaf_start, w.piece_base);
     hipLaunchKernelGGL(leaf_count_kernel, dim3(std::max<uint32_t>(1, 2 * w.num_cus)), dim3(kThreads), 0, stream,
                        leaf_in, n, d, g.leaves, (const uint64_t *)w.leaf_start, (const uint64_t *)w.piece_base, deg);
+    return hipGetLastError();
+}
+
+size_t csr_work_bytes(uint64_t n, uint32_t d, uint32_t num_cus)
+{
+    const CsrGeometry g = csr_geometry(n, d, num_cus);
+    return 4 * align256(g.slots * 4) + 2 * align256(g.counts * 8) + align256(kScanParts * 8) + align256(2 * 4) +
+           align256(2 * 8);
+}
+
+hipError_t launch_in_csr(const int32_t *G, uint64_t n, uint32_t d, int64_t *in_ptr, int32_t *in_src, void *work,
+                         uint32_t num_cus, hipStream_t stream)
+{
+    const CsrGeometry g = csr_geometry(n, d, num_cus);
+    char *p = (char *)work;
+    auto take = [&](size_t bytes) {
+        char *q = p;
+        p += align256(bytes);
+        return (void *)q;
+    };
+    int32_t *ka = (int32_t *)take(g.slots * 4), *va = (int32_t *)take(g.slots * 4);
+    int32_t *kb = (int32_t *)take(g.slots * 4), *vb = (int32_t *)take(g.slots * 4);
+    DegreeWork w;
+    w.counts = (uint64_t *)take(g.counts * 8);
+    w.offsets = (uint64_t *)take(g.counts * 8);
+    w.partials = (uint64_t *)take(kScanParts * 8);
+    uint32_t *seg_b0 = (uint32_t *)take(2 * 4);
+    uint64_t *seg_start = (uint64_t *)take(2 * 8);
+    const uint64_t len = (uint64_t)kRadixBins * g.blocks;
+    const double inv_d = 1.0 / d;
+    hipError_t e = hipSuccess;
+    hipLaunchKernelGGL(plan_whole_kernel, dim3(1), dim3(64), 0, stream, seg_b0, seg_start, g.blocks, g.slots,
+                       (uint64_t *)nullptr);
+    const int32_t *kin = G, *vin = nullptr;
+    for (uint32_t pass = 0; pass < g.passes; ++pass) {
+        const PartPlan pl{seg_b0, seg_start, 1u, kRadixBins, pass * kRadixBits, kRadixBins - 1};
+        const bool last = pass + 1 == g.passes;
+        int32_t *kout = (pass & 1) ? kb : ka;
+        int32_t *vout = last ? in_src : ((pass & 1) ? vb : va);
+        hipLaunchKernelGGL(part_count_kernel, dim3(g.blocks), dim3(kThreads), 0, stream, kin, pl, n, w.counts,
+                           (uint64_t *)nullptr);
+        if ((e = scan(w.counts, len, w, stream)) != hipSuccess) return e;
+        hipLaunchKernelGGL(lsd_scatter_kernel, dim3(g.blocks), dim3(kThreads), 0, stream, kin, vin, kout, vout, pl, n,
+                           d, inv_d, 0ull, (const uint64_t *)w.offsets);
+        if (pass == 0)  // later passes cover the valid pairs only
+            hipLaunchKernelGGL(plan_total_kernel, dim3(1), dim3(64), 0, stream, seg_b0, seg_start, g.blocks,
+                               (const uint64_t *)(w.offsets + len));
+        kin = kout;
+        vin = vout;
+    }
+    hipLaunchKernelGGL(csr_ptr_kernel, dim3(grid_of(g.slots + 1, kThreads, 16384)), dim3(kThreads), 0, stream, kin,
+                       (const uint64_t *)(seg_start + 1), n, g.slots, in_ptr);
     return hipGetLastError();
 }
 

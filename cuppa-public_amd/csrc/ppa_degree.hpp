@@ -31,4 +31,11 @@ void degree_work_layout(void *base, uint64_t n, uint32_t d, uint32_t num_cus, De
 hipError_t launch_degree_partitioned(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, const DegreeWork &w,
                                      hipStream_t stream);
 
+// In-edge CSR of all n*d slots (seed rows included): in_src[in_ptr[v] ..
+// in_ptr[v+1]) are the sources of the slots naming v in ascending order;
+// in_ptr[n] = valid slots. work holds csr_work_bytes(n, d, num_cus) bytes.
+size_t csr_work_bytes(uint64_t n, uint32_t d, uint32_t num_cus);
+hipError_t launch_in_csr(const int32_t *G, uint64_t n, uint32_t d, int64_t *in_ptr, int32_t *in_src, void *work,
+                         uint32_t num_cus, hipStream_t stream);
+
 }  // namespace ppa

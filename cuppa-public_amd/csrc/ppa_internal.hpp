@@ -18,8 +18,8 @@ struct ppa_ctx {
     ppa::DeviceScratch *scratch = nullptr;  // device
     uint32_t *degree_scratch = nullptr;     // device, grown on demand
     uint64_t degree_scratch_len = 0;
-    void *degree_work = nullptr;            // device buffers of the partitioned degree pass
-    size_t degree_work_bytes = 0;
+    void *work = nullptr;                   // device work area of the partition passes (degree, in-edge CSR)
+    size_t work_bytes = 0;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     ppa_stats last{};
     bool pending = false;        // a launch awaits ppa_ctx_sync / ppa_batch_sync
@@ -69,6 +69,10 @@ struct LaunchShape {
 // Grid, workgroup and claim sizes for `work` generated nodes with out-degree
 // d (d = 0: the batch kernel).
 int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *out);
+
+// Grows the context's work area to `need` bytes (waiting for `stream` before
+// freeing the old one); PPA_ERR_NOMEM if the device cannot provide it.
+int ensure_work(ppa_ctx *ctx, size_t need, hipStream_t stream);
 
 // Host image of a batch's device tables, packed as
 // BatchGraph[count] | slot_base[count+1] | row_base[count+1] | seg_base[nseg+1] |

@@ -104,6 +104,7 @@ _SIGNATURES = [
     ("ppa_ctx_sync", ctypes.c_int, [_VP, _VP]),
     ("ppa_degree_histogram", ctypes.c_int,
      [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP, ctypes.c_uint64, _VP, _VP]),
+    ("ppa_in_csr", ctypes.c_int, [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP, _VP]),
     ("ppa_batch_create", ctypes.c_int,
      [_VP, ctypes.POINTER(GraphDesc), ctypes.c_uint64, _VP, _VP, ctypes.POINTER(_VP)]),
     ("ppa_batch_launch", ctypes.c_int, [_VP, _VP]),
@@ -259,6 +260,10 @@ class Context:
         self.generate_into(desc, targets, rp, stream)
         return Graph(n, d, targets, rp)
 
+    def in_csr(self, g: Graph, stream=None):
+        """In-edge CSR (ppa_in_csr): (in_ptr int64 [n+1], in_src int32 [in_ptr[n]])."""
+        return _in_csr(self, g, stream)
+
     def degree_histogram(self, g: Graph, hist_len: int | None = None, want_degree: bool = False, stream=None):
         """Returns (hist uint64 tensor, max_degree int, degree tensor or None)."""
         import torch
@@ -280,6 +285,18 @@ class Context:
                                           ctypes.c_void_p(mx.data_ptr()), _stream_handle(stream)),
                "ppa_degree_histogram")
         return hist, int(mx.item()), deg
+
+
+def _in_csr(ctx, g: "Graph", stream=None):
+    import torch
+    dev = torch.device("cuda", ctx.device)
+    desc = make_desc(g.n, g.d, p_thresh=0)
+    in_ptr = torch.empty(g.n + 1, dtype=torch.int64, device=dev)
+    in_src = torch.empty(g.n * g.d, dtype=torch.int32, device=dev)
+    _check(lib().ppa_in_csr(ctx._h, ctypes.byref(desc), ctypes.c_void_p(g.targets.data_ptr()),
+                            ctypes.c_void_p(in_ptr.data_ptr()), ctypes.c_void_p(in_src.data_ptr()),
+                            _stream_handle(stream)), "ppa_in_csr")
+    return in_ptr, in_src[: int(in_ptr[-1].item())]
 
 
 def batch_layout(specs):

@@ -143,6 +143,17 @@ int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t
                          uint32_t *d_degree, uint64_t *d_hist, uint64_t hist_len,
                          uint64_t *d_max_degree, void *stream);
 
+/* ---- in-edge CSR (no counterpart in the reference, which never builds one;
+ *      the consumer of `targets` that the reference's users build next) ---- */
+/* in_src[in_ptr[v] .. in_ptr[v+1]) = the sources t of every slot (seed rows
+ * included) naming v, in ascending order; d_in_ptr int64[n+1], d_in_src
+ * int32[n*d]; in_ptr[n] = slots holding a node id in [0, n). A stable radix
+ * sort on the device: needs 16*n*d bytes of work memory, which the context
+ * keeps for the next call (PPA_ERR_NOMEM if it cannot be had). Waits for the
+ * stream. */
+int ppa_in_csr(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_targets, int64_t *d_in_ptr,
+               int32_t *d_in_src, void *stream);
+
 /* ---- batched ensemble (no counterpart in the reference: it generates one
  *      graph per process run; this is cuPPA::generateGraph over many graphs
  *      in one persistent launch) ------------------------------------------- */

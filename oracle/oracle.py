@@ -66,6 +66,8 @@ def lib():
     L.oracle_degrees.argtypes = [u64, u32, i32p, ctypes.POINTER(u64), ctypes.POINTER(u64), u64,
                                  ctypes.POINTER(u64)]
     L.oracle_degrees.restype = ctypes.c_int
+    L.oracle_in_csr.argtypes = [u64, u32, i32p, ctypes.POINTER(ctypes.c_int64), i32p]
+    L.oracle_in_csr.restype = ctypes.c_int
     L.oracle_verify_rows.argtypes = [u64, u32, u64, u64, i32p, u64, u64, ctypes.POINTER(u64)]
     L.oracle_verify_rows.restype = ctypes.c_int
     L.oracle_verify_parallel.argtypes = [u64, u32, u64, u64, i32p, u64, u64, ctypes.c_int,
@@ -189,6 +191,18 @@ def degrees(n: int, d: int, G: np.ndarray, hist_len: int = 0):
     if rc != 0:
         raise RuntimeError(f"oracle_degrees failed: {rc}")
     return deg, (hist[:hist_len] if hist_len else None), int(mx.value)
+
+
+def in_csr(n: int, d: int, G: np.ndarray):
+    """Returns (in_ptr int64 [n+1], in_src int32 [in_ptr[n]]) of all n*d slots."""
+    G = np.ascontiguousarray(G, dtype=np.int32)
+    in_ptr = np.empty(n + 1, dtype=np.int64)
+    in_src = np.empty(n * d, dtype=np.int32)
+    rc = lib().oracle_in_csr(n, d, _ptr(G, ctypes.c_int32), _ptr(in_ptr, ctypes.c_int64),
+                             _ptr(in_src, ctypes.c_int32))
+    if rc != 0:
+        raise RuntimeError(f"oracle_in_csr failed: {rc}")
+    return in_ptr, in_src[:in_ptr[n]]
 
 
 # ---------------------------------------------------------------------------

@@ -30,7 +30,9 @@
  * known-answer vectors and an independent Python big-integer restatement;
  * the mappings against Python big integers; generation against closed-form
  * special cases (p = 1, p = 0, d = 1), invariants and a pure-Python
- * restatement on small inputs; the closure oracle against the sequential one.
+ * restatement on small inputs; the closure oracle against the sequential one;
+ * the in-edge CSR against an independent stable argsort and against the
+ * degree pass.
  */
 #include <pthread.h>
 #include <stdint.h>
@@ -322,6 +324,28 @@ int oracle_degrees(uint64_t n, uint32_t d, const int32_t *G, uint64_t *deg_out,
     }
     if (max_degree) *max_degree = mx;
     if (!deg_out) free(deg);
+    return ORACLE_OK;
+}
+
+/* ---- in-edge CSR ---------------------------------------------------------- */
+/*
+ * Counting sort of all n*d slots (seed rows included) by target, in slot
+ * order: in_src[in_ptr[v] .. in_ptr[v+1]) are the sources t of the slots
+ * naming v, ascending. Slots outside [0, n) are skipped. No reference
+ * counterpart (the reference never builds in-edge lists).
+ */
+int oracle_in_csr(uint64_t n, uint32_t d, const int32_t *G, int64_t *in_ptr, int32_t *in_src)
+{
+    const uint64_t slots = n * (uint64_t)d;
+    int64_t *cur = (int64_t *)calloc(n + 1, sizeof(int64_t));
+    if (!cur) return ORACLE_ERR_NOMEM;
+    for (uint64_t s = 0; s < slots; ++s)
+        if (G[s] >= 0 && (uint64_t)G[s] < n) cur[G[s] + 1]++;
+    for (uint64_t v = 0; v < n; ++v) cur[v + 1] += cur[v];
+    for (uint64_t v = 0; v <= n; ++v) in_ptr[v] = cur[v];
+    for (uint64_t s = 0; s < slots; ++s)
+        if (G[s] >= 0 && (uint64_t)G[s] < n) in_src[cur[G[s]]++] = (int32_t)(s / d);
+    free(cur);
     return ORACLE_OK;
 }
 

@@ -1,0 +1,39 @@
+"""In-edge CSR on the GPU (ppa_in_csr) against the oracle's counting sort:
+generated graphs of one to four radix passes (n up to 2^24 + 5), and a
+synthetic target array with out-of-range values that must be skipped."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,d,p,seed", [
+    (200, 3, 0.5, 1),            # one 8-bit pass
+    (60_000, 5, 0.2, 2),         # two passes
+    (3_000_000, 8, 0.5, 3),      # three passes
+    ((1 << 24) + 5, 2, 0.1, 4),  # four passes
+])
+def test_in_csr_matches_oracle(ctx, oracle_mod, n, d, p, seed):
+    T = oracle_mod.p_threshold(p)
+    g = ctx.generate(n, d, p_thresh=T, seed=seed)
+    G = g.targets.cpu().numpy()
+    want_ptr, want_src = oracle_mod.in_csr(n, d, G)
+    ptr, src = ctx.in_csr(g)
+    assert np.array_equal(ptr.cpu().numpy(), want_ptr)
+    assert np.array_equal(src.cpu().numpy(), want_src)
+
+
+def test_in_csr_skips_out_of_range(ctx, oracle_mod):
+    import torch
+    import cuppa
+    n, d = 700_000, 6
+    rng = np.random.default_rng(9)
+    G = (np.arange(n * d) // d * rng.random(n * d) ** 3).astype(np.int32)
+    bad = rng.integers(0, n * d, 40)
+    G[bad[:20]] = -1
+    G[bad[20:]] = n + 3
+    g = cuppa.Graph(n, d, torch.from_numpy(G).cuda(), None)
+    want_ptr, want_src = oracle_mod.in_csr(n, d, G)
+    ptr, src = ctx.in_csr(g)
+    assert np.array_equal(ptr.cpu().numpy(), want_ptr)
+    assert np.array_equal(src.cpu().numpy(), want_src)

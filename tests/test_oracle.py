@@ -217,3 +217,32 @@ def test_verifier_accepts_oracle_and_catches_planted_mistakes(oracle_mod):
     seedbad = G.copy()
     seedbad[0] = 0
     assert oracle_mod.verify_rows(n, d, T, seed, seedbad, 0, d + 1) == 0
+
+
+def test_in_csr_oracle_pinned(oracle_mod):
+    """The in-edge CSR oracle (a counting sort) against an independent method
+    (numpy's stable argsort of the targets) and against the degree pass:
+    in(v) = deg(v) - d, plus d for the seed nodes 0..d."""
+    n, d = 5000, 4
+    T = oracle_mod.p_threshold(0.3)
+    G, _, _ = oracle_mod.generate(n, d, T, 3)
+    ptr, src = oracle_mod.in_csr(n, d, G)
+    order = np.argsort(G, kind="stable")
+    assert np.array_equal(src, (order // d).astype(np.int32))
+    assert np.array_equal(np.diff(ptr), np.bincount(G, minlength=n))
+    deg, _, _ = oracle_mod.degrees(n, d, G)
+    want_in = deg.astype(np.int64) - d
+    want_in[: d + 1] += d
+    assert np.array_equal(np.diff(ptr), want_in)
+    for v in (0, 1, d + 1, n // 2):  # each list ascending and naming v
+        lst = src[ptr[v]:ptr[v + 1]]
+        assert np.all(np.diff(lst) > 0)
+        assert all(v in G[t * d:(t + 1) * d] for t in lst)
+    # out-of-range values are skipped
+    H = G.copy()
+    H[[10, 20]] = -1
+    H[30] = n
+    ptr2, src2 = oracle_mod.in_csr(n, d, H)
+    assert ptr2[n] == n * d - 3
+    keep = (H >= 0) & (H < n)
+    assert np.array_equal(src2, (np.argsort(np.where(keep, H, n), kind="stable")[: ptr2[n]] // d).astype(np.int32))
