@@ -62,6 +62,7 @@ constexpr int kNt = 2;                    // cache policy: non-temporal (nt)
 constexpr uint64_t kPendingLimit = 0x7ffffffeull;
 constexpr int32_t kFarPair = 0x7fffffff;  // a copy source at or beyond kPendingLimit
 constexpr int kWaitVm0 = 0x0F70;          // s_waitcnt vmcnt(0), other counters untouched
+constexpr int kWaitLgkm0 = 0xC07F;        // s_waitcnt lgkmcnt(0), other counters untouched
 __device__ __forceinline__ int32_t pending_code(uint64_t src) { return (int32_t)(-2 - (int64_t)src); }
 __device__ __forceinline__ uint64_t pending_src(int32_t v) { return (uint64_t)(uint32_t)(-2 - v); }
 
@@ -320,12 +321,15 @@ void generate_kernel(GenArgs a)
         // a = i*d + kk (node i, slot kk) in pass a / 64 -- ceil(f*d / 64)
         // Philox passes where per-lane draws would cost d. A copy read goes
         // straight into word a of this wavefront's LDS stage (global_load_lds:
-        // no register waits on it, so every read of every pass is in flight
-        // together); word a of `info` keeps ~node for a direct draw, else the
-        // source pair (kFarPair when it cannot be kept as a pending code).
+        // no register waits on it); word a of `info` keeps ~node for a direct
+        // draw, else the source pair (kFarPair when it cannot be kept as a
+        // pending code). The assigned lanes then copy their info words into
+        // their value registers, which frees `info` for the re-reads below:
+        // both sets of reads are in flight together and the pass waits once.
         const uint64_t fresh_mask = __ballot(fresh);
+        uint32_t fd = 0, first = 0;
         if (fresh_mask != 0) {
-            const uint32_t fd = BATCH ? wd : (uint32_t)DMAX;
+            fd = BATCH ? wd : (uint32_t)DMAX;
             const uint32_t total = (uint32_t)__popcll(fresh_mask) * fd;
             const uint32_t magic = (65536u + fd - 1u) / fd;  // exact a / fd for a < 1024, fd <= 16
 #pragma unroll 1
@@ -346,30 +350,17 @@ void generate_kernel(GenArgs a)
                     }
                 }
             }
-            __builtin_amdgcn_s_waitcnt(kWaitVm0);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (fresh) {
-                const uint32_t first = rank_below(fresh_mask) * fd;
+                first = rank_below(fresh_mask) * fd;
 #pragma unroll
-                for (int kk = 0; kk < DMAX; ++kk) {
-                    if (!BATCH || (uint32_t)kk < fd) {
-                        const int32_t inf = info[first + kk];
-                        int32_t v = ~inf;
-                        if (inf >= 0) {
-                            v = stage[first + kk];
-                            // slot values are >= -1; anything else cannot be a read
-                            if (v <= kEmpty) {
-                                // an empty source is remembered in the slot itself
-                                ++c_pmiss;
-                                v = inf != kFarPair ? pending_code((uint32_t)inf) : kEmpty;
-                            }
-                        }
-                        val[kk] = v;
-                    }
-                }
+                for (int kk = 0; kk < DMAX; ++kk)
+                    if (!BATCH || (uint32_t)kk < fd) val[kk] = info[first + kk];
             }
+            // these LDS reads complete before any re-read below may land in info
+            __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
             __builtin_amdgcn_wave_barrier();
         }
 
@@ -378,20 +369,19 @@ void generate_kernel(GenArgs a)
         auto node_of = [&](uint64_t pair) -> int32_t {
             return (int32_t)(BATCH ? div_small(pair, dd, inv_d) : pair / (uint64_t)DMAX);
         };
-        if (fresh) {
-            // attempt 0 of every slot arrived through the stage above
-            c_draws += dd;
-            k = 0;
-            r = 0;
-            skip = 0;
-            misses = 0;
-            need_draw = false;
-            have_src = false;
-            fresh = false;
-        } else if (active) {
-            // One redraw or one coherent re-read for the current slot k.
-            int32_t got = kEmpty;
-            bool resolved = false;
+
+        // ---- redraws and the re-reads of still-empty sources (lanes not just assigned) ----
+        // One coherent re-read per pass for the source of a redrawn slot k, and
+        // for the attempt-0 sources still empty: slot k every pass (batches:
+        // with a back-off), the later ones every 4th or 16th pass. A lane's
+        // re-reads land in rows kk of the LDS info area (a redrawn slot k is
+        // never also pending, so its row is free); acceptance stays in slot
+        // order, but the waits of a node's slots overlap.
+        int32_t got = kEmpty;
+        bool resolved = false, lpoll = false;
+        uint32_t pmask = 0;
+        const bool wide = (iter & a.wide_mask) == 0u;
+        if (active && !fresh) {
             if (need_draw) {
                 const Attempt at = attempt(k0, k1, T, t, k, r, td);
                 ++c_draws;
@@ -404,14 +394,6 @@ void generate_kernel(GenArgs a)
                     have_src = true;
                 }
             }
-            // Coherent re-reads, one round trip per pass: the source of a
-            // redrawn slot k (row 0 of the LDS info area), and the attempt-0
-            // sources still empty -- slot k every pass (batches: with a
-            // back-off), the later ones every 4th or 16th pass -- in row kk of the
-            // LDS stage. Acceptance stays in slot order, but the waits of a
-            // node's slots overlap. Agent-scope global_load_lds: no register
-            // holds a read in flight.
-            bool lpoll = false;
             if (BATCH && have_src && skip > 0) {
                 // batches only: thousands of graphs' early rounds would
                 // otherwise flood memory with re-reads
@@ -419,31 +401,62 @@ void generate_kernel(GenArgs a)
             } else if (have_src) {
                 lpoll = true;
             }
-            const bool wide = (iter & a.wide_mask) == 0u;
             const bool head_ok = !(BATCH && skip > 0);
-            uint32_t pmask = 0;
 #pragma unroll
             for (int kk = 0; kk < DMAX; ++kk)
                 if ((uint32_t)kk >= k && (!BATCH || (uint32_t)kk < dd) && val[kk] < kEmpty &&
                     ((uint32_t)kk == k ? head_ok : wide))
                     pmask |= 1u << kk;
-            if (__ballot(lpoll || pmask != 0) != 0) {
-                if (lpoll)
-                    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(G + src),
-                                                     (__attribute__((address_space(3))) void *)info, 4, 0, kSc1);
+        }
+        const bool any_read = __ballot(lpoll || pmask != 0) != 0;
+        if (any_read) {
 #pragma unroll
-                for (int kk = 0; kk < DMAX; ++kk) {
-                    if ((pmask >> kk) & 1u)
-                        __builtin_amdgcn_global_load_lds(
-                            (const __attribute__((address_space(1))) void *)(G + pending_src(val[kk])),
-                            (__attribute__((address_space(3))) void *)(stage + kk * 64), 4, 0, kSc1);
-                }
-                c_polls += (uint32_t)__popc(pmask) + (lpoll ? 1u : 0u);
-                __builtin_amdgcn_s_waitcnt(kWaitVm0);
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int kk = 0; kk < DMAX; ++kk) {
+                const bool legacy = lpoll && (uint32_t)kk == k;
+                if (legacy || ((pmask >> kk) & 1u))
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void *)(G + (legacy ? src : pending_src(val[kk]))),
+                        (__attribute__((address_space(3))) void *)(info + kk * 64), 4, 0, kSc1);
             }
+            c_polls += (uint32_t)__popc(pmask) + (lpoll ? 1u : 0u);
+        }
+        if (fresh_mask != 0 || any_read) {
+            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+
+        if (fresh) {
+            // attempt 0 of every slot: val holds ~node or the source pair, the
+            // stage the value read from a copy source
+#pragma unroll
+            for (int kk = 0; kk < DMAX; ++kk) {
+                if (!BATCH || (uint32_t)kk < fd) {
+                    const int32_t inf = val[kk];
+                    int32_t v = ~inf;
+                    if (inf >= 0) {
+                        v = stage[first + kk];
+                        // slot values are >= -1; anything else cannot be a read
+                        if (v <= kEmpty) {
+                            // an empty source is remembered in the slot itself
+                            ++c_pmiss;
+                            v = inf != kFarPair ? pending_code((uint32_t)inf) : kEmpty;
+                        }
+                    }
+                    val[kk] = v;
+                }
+            }
+            c_draws += dd;
+            k = 0;
+            r = 0;
+            skip = 0;
+            misses = 0;
+            need_draw = false;
+            have_src = false;
+            fresh = false;
+        } else if (active) {
             if (lpoll) {
-                const int32_t v = info[lane];
+                const int32_t v = info[k * 64u + lane];
                 if (v > kEmpty) {
                     got = v;
                     resolved = true;
@@ -466,7 +479,7 @@ void generate_kernel(GenArgs a)
             for (int kk = 0; kk < DMAX; ++kk) {
                 if ((uint32_t)kk >= k && (!BATCH || (uint32_t)kk < dd) && val[kk] < kEmpty) {
                     const bool head = (uint32_t)kk == k;
-                    const int32_t got_p = (pmask >> kk) & 1u ? stage[kk * 64 + lane] : kEmpty;
+                    const int32_t got_p = (pmask >> kk) & 1u ? info[kk * 64 + lane] : kEmpty;
                     if (got_p > kEmpty) {
                         val[kk] = got_p;
                         progressed = true;
@@ -484,8 +497,8 @@ void generate_kernel(GenArgs a)
                     }
                 }
             }
-            __builtin_amdgcn_wave_barrier();
         }
+        __builtin_amdgcn_wave_barrier();
 
         // ---- accept slots in order: duplicate check against filled slots ----
         bool finished = false, retried = false;

@@ -118,6 +118,8 @@ void printUsage(const char *prog)
         "  --device D                GPU ordinal\n"
         "  --chunk C                 nodes per wavefront claim (multiple of 64)\n"
         "  --edges-file PATH         write the edge list (\"u v\" lines)\n"
+        "  --csr-file PATH           write the out-edge CSR (binary, see writeCsr)\n"
+        "  --in-csr-file PATH        write the in-edge CSR (binary; cuppa paths)\n"
         "  -o,--output-dir DIR       output directory\n",
         prog);
 }
@@ -161,6 +163,8 @@ int parseExperiments(int argc, char **argv, ExperimentConfiguration *e, std::str
         {"--device", i32(&e->_device)},
         {"--chunk", u32(&e->_chunk)},
         {"--edges-file", str(&e->_edgesFile)},
+        {"--csr-file", str(&e->_csrFile)},
+        {"--in-csr-file", str(&e->_inCsrFile)},
         {"-o", str(&e->_outputDir)}, {"--output-dir", str(&e->_outputDir)},
     };
     for (int i = 1; i < argc; ++i) {

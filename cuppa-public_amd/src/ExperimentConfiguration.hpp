@@ -6,7 +6,7 @@
 // [config] INI keys of ExperimentConfiguration.hpp:114-189. The parser and the
 // INI reader are this project's own (the reference vendors CLI11 and minIni).
 // Added: --seed (Philox key; the reference seeds with time(0), cuPPA_1.cu:39),
-// --device, --chunk and --edges-file.
+// --device, --chunk, --edges-file, --csr-file and --in-csr-file.
 #pragma once
 
 #include <cstdint>
@@ -55,6 +55,8 @@ struct ExperimentConfiguration {
     int _device = 0;
     uint32_t _chunk = 0;
     std::string _edgesFile;
+    std::string _csrFile;    // binary out-edge CSR (row offsets + targets)
+    std::string _inCsrFile;  // binary in-edge CSR (GPU paths)
 
     // INI loader: every key of the [config] section with the reference's
     // defaults for missing keys (ExperimentConfiguration.hpp:114-189).

@@ -3,7 +3,9 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -37,6 +39,73 @@ bool writeEdgeList(const std::string &path, const int32_t *G, uint64_t n, uint32
         for (uint32_t k = 0; k < d; ++k) std::fprintf(f, "%llu %d\n", (unsigned long long)t, G[t * d + k]);
     const bool ok = std::ferror(f) == 0;
     return std::fclose(f) == 0 && ok;
+}
+
+namespace {
+
+// Writes `bytes` from device memory, double-buffered through pinned memory.
+bool writeDevice(FILE *f, const void *dptr, size_t bytes)
+{
+    constexpr size_t kChunk = 64u << 20;
+    void *buf[2] = {nullptr, nullptr};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool ok = hipHostMalloc(&buf[0], kChunk, hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc(&buf[1], kChunk, hipHostMallocDefault) == hipSuccess &&
+              hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) == hipSuccess;
+    const char *src = (const char *)dptr;
+    const size_t chunks = (bytes + kChunk - 1) / kChunk;
+    auto issue = [&](size_t c) {
+        const size_t off = c * kChunk, len = std::min(kChunk, bytes - off);
+        return hipMemcpyAsync(buf[c & 1], src + off, len, hipMemcpyDeviceToHost, stream) == hipSuccess &&
+               hipEventRecord(ev[c & 1], stream) == hipSuccess;
+    };
+    if (ok && chunks) ok = issue(0);
+    for (size_t c = 0; ok && c < chunks; ++c) {
+        ok = hipEventSynchronize(ev[c & 1]) == hipSuccess;
+        if (ok && c + 1 < chunks) ok = issue(c + 1);  // next copy runs while this chunk is written
+        const size_t len = std::min(kChunk, bytes - c * kChunk);
+        if (ok) ok = std::fwrite(buf[c & 1], 1, len, f) == len;
+    }
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (int i = 0; i < 2; ++i) {
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+        if (buf[i]) (void)hipHostFree(buf[i]);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+    return ok;
+}
+
+}  // namespace
+
+bool writeCsr(const std::string &path, const CsrHeader &h, const int64_t *offsets, const int32_t *indices,
+              bool device)
+{
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+    const size_t ob = (h.n + 1) * sizeof(int64_t), ib = h.nnz * sizeof(int32_t);
+    if (device) {
+        ok = ok && writeDevice(f, offsets, ob) && writeDevice(f, indices, ib);
+    } else {
+        ok = ok && std::fwrite(offsets, 1, ob, f) == ob && std::fwrite(indices, 1, ib, f) == ib;
+    }
+    return std::fclose(f) == 0 && ok;
+}
+
+CsrHeader csrHeader(const ppa_graph_desc &g, uint64_t nnz, uint32_t kind)
+{
+    CsrHeader h{};
+    std::memcpy(h.magic, "PPACSR1", 8);
+    h.n = g.n;
+    h.nnz = nnz;
+    h.d = g.d;
+    h.kind = kind;
+    h.seed = g.seed;
+    h.p_threshold = g.p_threshold;
+    return h;
 }
 
 // ---- cuPPA ------------------------------------------------------------------
@@ -114,6 +183,26 @@ int cuPPA::run(const ExperimentConfiguration &conf, const char *timingCsv)
             return PPA_ERR_DEVICE;
         if (!writeEdgeList(conf._edgesFile, host.data(), g.n, g.d)) return PPA_ERR_INVALID;
     }
+    if (!conf._csrFile.empty() &&
+        !writeCsr(conf._csrFile, csrHeader(g, g.n * g.d, 0), _dRowPtr, _dGraph, true))
+        return PPA_ERR_INVALID;
+    if (!conf._inCsrFile.empty()) {
+        int64_t *dPtr = nullptr;
+        int32_t *dSrc = nullptr;
+        rc = hipMalloc(&dPtr, (g.n + 1) * sizeof(int64_t)) == hipSuccess &&
+                     hipMalloc(&dSrc, g.n * g.d * sizeof(int32_t)) == hipSuccess
+                 ? PPA_OK
+                 : PPA_ERR_NOMEM;
+        if (!rc) rc = ppa_in_csr(_ctx, &g, _dGraph, dPtr, dSrc, nullptr);
+        int64_t nnz = 0;
+        if (!rc && hipMemcpy(&nnz, dPtr + g.n, sizeof(nnz), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = PPA_ERR_DEVICE;
+        if (!rc && !writeCsr(conf._inCsrFile, csrHeader(g, (uint64_t)nnz, 1), dPtr, dSrc, true))
+            rc = PPA_ERR_INVALID;
+        if (dPtr) (void)hipFree(dPtr);
+        if (dSrc) (void)hipFree(dSrc);
+        if (rc) return rc;
+    }
     std::cout << "Ended" << std::endl;
     return PPA_OK;
 }
@@ -181,5 +270,11 @@ int SerialCopyModel::generateGraph(const ExperimentConfiguration &conf)
         if (!writeDegreeCsv(joinPath(conf._outputDir, conf._distfile1), hist)) return PPA_ERR_INVALID;
     }
     if (!conf._edgesFile.empty() && !writeEdgeList(conf._edgesFile, _G.data(), g.n, g.d)) return PPA_ERR_INVALID;
+    if (!conf._csrFile.empty()) {
+        std::vector<int64_t> rowPtr(g.n + 1);
+        for (uint64_t t = 0; t <= g.n; ++t) rowPtr[t] = (int64_t)(t * g.d);
+        if (!writeCsr(conf._csrFile, csrHeader(g, g.n * g.d, 0), rowPtr.data(), _G.data(), false))
+            return PPA_ERR_INVALID;
+    }
     return PPA_OK;
 }

@@ -66,3 +66,22 @@ bool writeDegreeCsv(const std::string &path, const std::vector<uint64_t> &hist);
 // seed clique's edges once each, then every slot of nodes t >= d+1.
 bool writeEdgeList(const std::string &path, const int32_t *targets, uint64_t n, uint32_t d);
 std::string joinPath(const std::string &dir, const std::string &file);
+
+// Binary CSR file (this project's format; the reference writes text only):
+// a 64-byte little-endian header
+//   char magic[8] = "PPACSR1"; uint64 n; uint64 nnz; uint32 d; uint32 kind
+//   (0 = out-edges: row t lists t's targets; 1 = in-edges: row v lists the
+//   sources naming v, ascending); uint64 seed; uint64 p_threshold; uint64 0, 0
+// then int64 offsets[n + 1] and int32 indices[nnz]. Arrays given as device
+// pointers are streamed through pinned staging buffers, the copy of one chunk
+// overlapping the write of the previous one.
+struct CsrHeader {
+    char magic[8];
+    uint64_t n, nnz;
+    uint32_t d, kind;
+    uint64_t seed, p_threshold;
+    uint64_t reserved[2];
+};
+static_assert(sizeof(CsrHeader) == 64, "CSR header layout");
+bool writeCsr(const std::string &path, const CsrHeader &h, const int64_t *offsets, const int32_t *indices,
+              bool device);

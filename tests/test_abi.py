@@ -171,3 +171,29 @@ def test_runner_rejects_bad_input(runner, tmp_path):
     assert r.returncode == 1 and "invalid argument" in r.stderr
     r = run(["-n", "abc"], tmp_path)
     assert r.returncode == 1
+
+
+def read_csr(path):
+    """Reads the driver's binary CSR file: 64-byte header, int64 offsets[n+1],
+    int32 indices[nnz]."""
+    raw = open(path, "rb").read()
+    magic = raw[:8]
+    n, nnz, d, kind, seed, thr = np.frombuffer(raw[8:48], dtype=np.dtype(
+        [("n", "<u8"), ("nnz", "<u8"), ("d", "<u4"), ("kind", "<u4"), ("seed", "<u8"), ("thr", "<u8")]))[0]
+    ptr = np.frombuffer(raw, dtype="<i8", count=int(n) + 1, offset=64)
+    idx = np.frombuffer(raw, dtype="<i4", count=int(nnz), offset=64 + 8 * (int(n) + 1))
+    assert len(raw) == 64 + 8 * (int(n) + 1) + 4 * int(nnz)
+    return magic, int(n), int(nnz), int(d), int(kind), int(seed), int(thr), ptr, idx
+
+
+def test_runner_scm_csr_file(runner, tmp_path, oracle_mod):
+    n, d, p, seed = 20_000, 3, 0.4, 8
+    r = run(["-A", "scm", "-n", str(n), "-x", str(d), "-a", str(p), "--seed", str(seed),
+             "--csr-file", str(tmp_path / "g.csr")], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    magic, n2, nnz, d2, kind, seed2, thr, ptr, idx = read_csr(tmp_path / "g.csr")
+    T = oracle_mod.p_threshold(p)
+    G, _, _ = oracle_mod.generate(n, d, T, seed)
+    assert magic == b"PPACSR1\0" and (n2, nnz, d2, kind, seed2, thr) == (n, n * d, d, 0, seed, T)
+    assert np.array_equal(ptr, np.arange(n + 1, dtype=np.int64) * d)
+    assert np.array_equal(idx, G)

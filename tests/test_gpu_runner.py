@@ -39,3 +39,25 @@ def test_runner_gpu_outputs(tmp_path, oracle_mod, algo):
     lines = (tmp_path / csv).read_text().splitlines()
     assert lines[0] == "Round,nBlocks,nThreads,Edges,MaxQ,Initialization Time,Computation Time"
     assert len(lines) == 2 and lines[1].split(",")[3] == str(n * d)
+
+
+def test_runner_gpu_csr_files(tmp_path, oracle_mod):
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_abi import read_csr
+    n, d, p, seed = 40_000, 5, 0.25, 13
+    r = subprocess.run([RUNNER, "-A", "cuppa", "-n", str(n), "-x", str(d), "-a", str(p), "--seed", str(seed),
+                        "--csr-file", str(tmp_path / "out.csr"), "--in-csr-file", str(tmp_path / "in.csr")],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    T = oracle_mod.p_threshold(p)
+    G, _, _ = oracle_mod.generate(n, d, T, seed)
+    magic, n2, nnz, d2, kind, seed2, thr, ptr, idx = read_csr(tmp_path / "out.csr")
+    assert (magic, n2, nnz, d2, kind, seed2, thr) == (b"PPACSR1\0", n, n * d, d, 0, seed, T)
+    assert np.array_equal(ptr, np.arange(n + 1, dtype=np.int64) * d)
+    assert np.array_equal(idx, G)
+    want_ptr, want_src = oracle_mod.in_csr(n, d, G)
+    magic, n2, nnz, d2, kind, _, _, ptr, idx = read_csr(tmp_path / "in.csr")
+    assert (n2, nnz, kind) == (n, n * d, 1)
+    assert np.array_equal(ptr, want_ptr)
+    assert np.array_equal(idx, want_src)
