@@ -37,7 +37,7 @@ namespace ppa {
 size_t batch_tables_bytes(uint64_t count)
 {
     return count * sizeof(BatchGraph) + (2 * (count + 1) + (count + 1) + 2 * count) * sizeof(uint64_t) +
-           count * sizeof(uint32_t);
+           2 * count * sizeof(uint32_t);
 }
 
 BatchGraph graph_record(const ppa_graph_desc &dsc)
@@ -59,6 +59,7 @@ struct TableView {
     BatchGraph *graphs;
     uint64_t *slot, *row, *seg_base, *seg_round, *seg_active;
     uint32_t *order;
+    uint32_t *done;  // finished nodes per graph
 };
 
 TableView view_of(char *base, uint64_t count, uint64_t nseg)
@@ -71,6 +72,7 @@ TableView view_of(char *base, uint64_t count, uint64_t nseg)
     v.seg_round = v.seg_base + (nseg + 1);
     v.seg_active = v.seg_round + nseg;
     v.order = (uint32_t *)(v.seg_active + nseg);
+    v.done = v.order + count;
     return v;
 }
 
@@ -107,7 +109,7 @@ void build_batch_tables(const std::vector<BatchGraph> &graphs, uint64_t chunk, B
     out->count = count;
     out->nseg = nseg;
     out->bytes.assign(count * sizeof(BatchGraph) + (2 * (count + 1) + (nseg + 1) + 2 * nseg) * sizeof(uint64_t) +
-                          count * sizeof(uint32_t),
+                          2 * count * sizeof(uint32_t),
                       0);
     TableView v = view_of(out->bytes.data(), count, nseg);
     std::memcpy(v.graphs, graphs.data(), count * sizeof(BatchGraph));
@@ -127,12 +129,12 @@ void build_batch_tables(const std::vector<BatchGraph> &graphs, uint64_t chunk, B
 }
 
 hipError_t enqueue_batch(void *d_tables, const BatchTables &t, const LaunchShape &shape, DeviceScratch *scratch,
-                         hipStream_t stream)
+                         hipStream_t stream, uint32_t *done_flag)
 {
     TableView v = view_of((char *)d_tables, t.count, t.nseg);
     hipError_t e = launch_reset_scratch(scratch, 0, stream);
     if (e != hipSuccess) return e;
-    BatchFillArgs fa{v.graphs, v.slot, v.row, t.count};
+    BatchFillArgs fa{v.graphs, v.slot, v.row, t.count, v.done};
     if ((e = launch_fill_batch(fa, t.total_slots, t.total_rows, stream)) != hipSuccess) return e;
     if (t.work == 0) return hipSuccess;
     GenArgs ga{};
@@ -151,6 +153,8 @@ hipError_t enqueue_batch(void *d_tables, const BatchTables &t, const LaunchShape
     ga.sleep_cap = shape.sleep_cap;
     ga.wide_mask = shape.wide_mask;
     ga.scratch = scratch;
+    ga.done_nodes = v.done;
+    ga.done_flag = done_flag;
     return launch_generate_batch(ga, shape.blocks, shape.threads, stream);
 }
 

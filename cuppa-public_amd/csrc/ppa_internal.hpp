@@ -91,8 +91,10 @@ size_t batch_tables_bytes(uint64_t count);
 // graphs[i].G / row_ptr / targets_offset must be set; the rest is derived here.
 void build_batch_tables(const std::vector<BatchGraph> &graphs, uint64_t chunk, BatchTables *out);
 // reset + fill + generate of a batch whose tables are already at d_tables.
+// done_flag (nullable, mapped host memory, one word per graph) is set to 1
+// for each graph as soon as it is complete.
 hipError_t enqueue_batch(void *d_tables, const BatchTables &t, const LaunchShape &shape, DeviceScratch *scratch,
-                         hipStream_t stream);
+                         hipStream_t stream, uint32_t *done_flag = nullptr);
 // A descriptor -> the kernel's graph record (G, row_ptr and offsets set by the caller).
 BatchGraph graph_record(const ppa_graph_desc &desc);
 // Waits for the stream and turns the device scratch into ctx->last.

@@ -161,6 +161,10 @@ __global__ __launch_bounds__(256) void fill_batch_kernel(BatchFillArgs a, uint64
                                                           uint64_t total_rows)
 {
     const uint64_t per_block = (uint64_t)blockDim.x * kFillPerThread;
+    if (a.done_nodes)
+        for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < a.count;
+             g += (uint64_t)gridDim.x * blockDim.x)
+            a.done_nodes[g] = 0;
     for (uint64_t base = blockIdx.x * per_block; base < total_slots; base += gridDim.x * per_block) {
         uint64_t g = find_segment(a.slot_base, a.count, base);
         for (uint32_t j = 0; j < kFillPerThread; ++j) {
@@ -609,6 +613,39 @@ void generate_kernel(GenArgs a)
 #pragma unroll
                 for (int kk = 0; kk < DMAX; ++kk)
                     if (!BATCH || (uint32_t)kk < dd) store_coherent(G + td + kk, val[kk]);
+            }
+        }
+
+        // ---- batches: per-graph completion ----
+        // After this pass's rows are published, the finished nodes are added
+        // to their graphs' counters, one atomic per graph present in the
+        // wavefront; whoever completes a graph raises its host flag, so a
+        // caller can use that graph before the whole batch ends. Every store
+        // that publishes a slot is an agent-scope atomic store (sc1), which is
+        // complete at agent scope once vmcnt drains, so a wait orders them
+        // before the count (a release fence would also write back the L2 every
+        // pass); the counter's atomic order then puts every wavefront's rows of
+        // a graph before the count that completes it.
+        if (BATCH && a.done_flag) {
+            uint64_t fm = __ballot(finished);
+            if (fm != 0) {
+                __builtin_amdgcn_s_waitcnt(kWaitVm0);
+                while (fm != 0) {
+                    const int leader = __builtin_ctzll(fm);
+                    const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)gid, leader);
+                    const uint64_t m = __ballot(finished && gid == g);
+                    if ((int)lane == leader) {
+                        const uint32_t c = (uint32_t)__popcll(m);
+                        const uint32_t old = __hip_atomic_fetch_add(&a.done_nodes[g], c, __ATOMIC_RELAXED,
+                                                                    __HIP_MEMORY_SCOPE_AGENT);
+                        const BatchGraph &gr = a.graphs[g];
+                        if (a.done_flag && (uint64_t)old + c == gr.n - gr.d - 1) {
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                            __hip_atomic_store(&a.done_flag[g], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                        }
+                    }
+                    fm &= ~m;
+                }
             }
         }
 

@@ -70,6 +70,10 @@ struct GenArgs {
     DeviceScratch *scratch;
     uint32_t sleep_cap;       // longest wavefront back-off, in 256-clock sleeps (>= 1)
     uint32_t wide_mask;       // later pending slots are re-read when (pass & wide_mask) == 0
+    // batch: finished nodes per graph (device), and per-graph flags in mapped
+    // host memory set to 1 once a graph is complete and published (nullable)
+    uint32_t *done_nodes;
+    uint32_t *done_flag;
 };
 
 struct FillArgs {
@@ -85,6 +89,7 @@ struct BatchFillArgs {
     const uint64_t *slot_base;  // exclusive prefix of n_i * d_i, count + 1 entries
     const uint64_t *row_base;   // exclusive prefix of n_i + 1, count + 1 entries
     uint64_t count;
+    uint32_t *done_nodes;       // per-graph finished-node counters, zeroed here
 };
 
 hipError_t launch_reset_scratch(DeviceScratch *s, uint64_t first, hipStream_t stream);

@@ -45,9 +45,13 @@ struct Slot {
     void *h_tables = nullptr;          // pinned staging for the table upload
     size_t table_cap = 0;
     hipEvent_t done = nullptr;
+    uint32_t *flags = nullptr;    // mapped, coherent host memory: graph i of a batch launch is complete
+    uint32_t *d_flags = nullptr;  // the same words as the device addresses them
     bool busy = false;
+    bool batch = false;
     uint64_t expect_nodes = 0;
     std::vector<uint64_t> ids;
+    std::vector<char> retired;    // ids[i] already finished through its flag
 };
 
 // First-fit allocator over [0, size) in granules.
@@ -133,6 +137,7 @@ void destroy_server(ppa_server *s)
         if (sl.status) (void)hipHostFree(sl.status);
         if (sl.d_tables) (void)hipFree(sl.d_tables);
         if (sl.h_tables) (void)hipHostFree(sl.h_tables);
+        if (sl.flags) (void)hipHostFree(sl.flags);
     }
     if (s->pool_base) (void)hipFree(s->pool_base);
     delete s;
@@ -211,8 +216,12 @@ int launch(ppa_server *s, Slot &sl, const std::vector<uint64_t> &ids)
         if (t.bytes.size() > sl.table_cap) return PPA_ERR_INTERNAL;
         std::memcpy(sl.h_tables, t.bytes.data(), t.bytes.size());
         PPA_HIP(hipMemcpyAsync(sl.d_tables, sl.h_tables, t.bytes.size(), hipMemcpyHostToDevice, sl.stream));
-        PPA_HIP(enqueue_batch(sl.d_tables, t, shape, sl.scratch, sl.stream));
+        // the slot's previous launch has ended, so its flags are ours to clear
+        std::memset(sl.flags, 0, ids.size() * sizeof(uint32_t));
+        PPA_HIP(enqueue_batch(sl.d_tables, t, shape, sl.scratch, sl.stream, sl.d_flags));
     }
+    sl.batch = !single;
+    sl.retired.assign(ids.size(), 0);
     PPA_HIP(hipMemcpyAsync(sl.status, sl.scratch, sizeof(DeviceScratch), hipMemcpyDeviceToHost, sl.stream));
     PPA_HIP(hipEventRecord(sl.done, sl.stream));
     const double now = s->now_ms();
@@ -262,7 +271,10 @@ int ppa_server_create(ppa_ctx *ctx, const ppa_server_config *cfg, ppa_server **o
             hipMalloc(&sl.scratch, sizeof(DeviceScratch)) != hipSuccess ||
             hipHostMalloc(&sl.status, sizeof(DeviceScratch), hipHostMallocDefault) != hipSuccess ||
             hipMalloc(&sl.d_tables, cap) != hipSuccess ||
-            hipHostMalloc(&sl.h_tables, cap, hipHostMallocDefault) != hipSuccess) {
+            hipHostMalloc(&sl.h_tables, cap, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc(&sl.flags, s->cfg.max_batch * sizeof(uint32_t),
+                          hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+            hipHostGetDevicePointer((void **)&sl.d_flags, sl.flags, 0) != hipSuccess) {
             destroy_server(s);
             return PPA_ERR_NOMEM;
         }
@@ -304,23 +316,41 @@ int ppa_server_step(ppa_server *s)
     DeviceGuard g(s->ctx->device);
     if (!g.ok()) return PPA_ERR_DEVICE;
     int finished = 0;
-    // 1. retire finished launches
+    // 1. retire finished requests: graphs of a running batch launch as their
+    //    flags rise (the kernel sets one only after the graph is published),
+    //    then whole launches as their events complete
     for (auto &sl : s->slots) {
         if (!sl.busy) continue;
+        if (sl.batch) {
+            double now = -1;
+            for (size_t i = 0; i < sl.ids.size(); ++i) {
+                if (sl.retired[i] || __atomic_load_n(&sl.flags[i], __ATOMIC_ACQUIRE) == 0) continue;
+                if (now < 0) now = s->now_ms();
+                Request &r = s->reqs[sl.ids[i]];
+                r.state = PPA_REQ_DONE;
+                r.status = PPA_OK;
+                r.done_ms = now;
+                sl.retired[i] = 1;
+                --s->running;
+                ++finished;
+            }
+        }
         const hipError_t q = hipEventQuery(sl.done);
         if (q == hipErrorNotReady) continue;
         int status = q == hipSuccess ? sl.status->status : PPA_ERR_DEVICE;
         if (status == PPA_OK && sl.status->stats[kStatNodes] != sl.expect_nodes) status = PPA_ERR_INTERNAL;
         const double now = s->now_ms();
-        for (uint64_t id : sl.ids) {
-            Request &r = s->reqs[id];
+        for (size_t i = 0; i < sl.ids.size(); ++i) {
+            if (sl.retired[i]) continue;  // complete before the launch ended: valid whatever the status
+            Request &r = s->reqs[sl.ids[i]];
             r.state = status == PPA_OK ? PPA_REQ_DONE : PPA_REQ_FAILED;
             r.status = status;
             r.done_ms = now;
+            --s->running;
             ++finished;
         }
-        s->running -= sl.ids.size();
         sl.ids.clear();
+        sl.retired.clear();
         sl.busy = false;
     }
     // 2. admit in arrival order while the pool has room

@@ -191,8 +191,9 @@ int ppa_autotune(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, int3
  * concurrent persistent launches, so requests that arrive while others run
  * overlap with them on the device. Results stay on the device until released.
  * A request's output equals ppa_generate of the same graph alone; its row_ptr
- * holds t*d (offsets into its own targets). Calls must not overlap (one host
- * thread drives a server). */
+ * holds t*d (offsets into its own targets). A request batched with others is
+ * reported DONE as soon as its own graph is complete, before the rest of its
+ * launch. Calls must not overlap (one host thread drives a server). */
 typedef struct ppa_server ppa_server;
 
 typedef struct ppa_server_config {
