@@ -129,14 +129,22 @@ ppa_graph_desc cuPPA::desc(const ExperimentConfiguration &conf) const
     return g;
 }
 
-int cuPPA::generateGraph(const ExperimentConfiguration &conf) { return run(conf, "StepwiseTiming-Original.csv"); }
-
-int cuPPA::generateGraphAdaptive(const ExperimentConfiguration &conf)
+int cuPPA::generateGraph(const ExperimentConfiguration &conf)
 {
-    return run(conf, "StepwiseTiming-Adaptive.csv");
+    return run(conf, "StepwiseTiming-Original.csv", false);
 }
 
-int cuPPA::run(const ExperimentConfiguration &conf, const char *timingCsv)
+// cuPPA_2.cu:181-198 doubles the launch width while its queues stay below
+// half capacity. One persistent launch has no stages, and every launch shape
+// yields the same graph, so the online form here tunes on the graph itself:
+// the persistent grid at 1/8, 1/4, 1/2 and all of its size, stopping when a
+// doubling gains < 3 %, one StepwiseTiming row per trial.
+int cuPPA::generateGraphAdaptive(const ExperimentConfiguration &conf)
+{
+    return run(conf, "StepwiseTiming-Adaptive.csv", true);
+}
+
+int cuPPA::run(const ExperimentConfiguration &conf, const char *timingCsv, bool adaptive)
 {
     const ppa_graph_desc g = desc(conf);
     int rc = ppa_check_desc(&g);
@@ -162,18 +170,48 @@ int cuPPA::run(const ExperimentConfiguration &conf, const char *timingCsv)
     ppa_stats st{};
     ppa_get_stats(_ctx, &st);
     if (rc) return rc;
+    std::vector<ppa_stats> trials{st};
+    if (adaptive && !conf._blocksGiven) {
+        // the first run's grid is the full one; try 1/8, 1/4, 1/2 of it upwards
+        const uint64_t full = st.grid_blocks;
+        ppa_stats best = st;
+        double prev = -1;
+        for (int shift = 3; shift >= 1; --shift) {
+            const uint64_t blocks = full >> shift;
+            if (blocks == 0) continue;
+            ppa_tuning t2 = tu;
+            t2.max_blocks = (uint32_t)blocks;
+            if ((rc = ppa_ctx_set_tuning(_ctx, &t2)) != PPA_OK) return rc;
+            if ((rc = ppa_generate(_ctx, &g, _dGraph, _dRowPtr, nullptr)) != PPA_OK) return rc;
+            ppa_stats s2{};
+            ppa_get_stats(_ctx, &s2);
+            trials.push_back(s2);
+            if (s2.kernel_ms < best.kernel_ms) best = s2;
+            if (prev > 0 && s2.kernel_ms > 0.97 * prev) break;  // doubling no longer pays
+            prev = s2.kernel_ms;
+        }
+        // the kept shape for later calls on this context
+        ppa_tuning t3 = tu;
+        t3.max_blocks = best.grid_blocks == full ? 0u : (uint32_t)best.grid_blocks;
+        if ((rc = ppa_ctx_set_tuning(_ctx, &t3)) != PPA_OK) return rc;
+        st = best;
+    }
     std::printf("BLOCKS: %llu\t| THREADS PER BLOCK: %llu\t| DRAWS: %llu\t| RETRIES: %llu\t| PARKED: %llu\n",
                 (unsigned long long)st.grid_blocks, (unsigned long long)st.block_threads,
                 (unsigned long long)st.draws, (unsigned long long)st.retries,
                 (unsigned long long)st.prefetch_misses);
     {
-        // cuPPA_1.cu:84-87,217-224: one row per stage; here a single stage.
+        // cuPPA_1.cu:84-87,217-224: one row per stage; here one per launch
+        // (the adaptive mode's trials, the plain mode's single launch).
         std::ofstream csv(timingCsv);
         csv << "Round,nBlocks,nThreads,Edges,MaxQ,Initialization Time,Computation Time\n";
-        if (conf._N_STAGE_TIMINGS > 0)
-            csv << 1 << "," << st.grid_blocks << "," << st.block_threads << "," << g.n * g.d << ","
-                << st.prefetch_misses << "," << (st.total_ms - st.kernel_ms) * 1e-3 << ","
-                << st.kernel_ms * 1e-3 << "\n";
+        const size_t rows = std::min<size_t>(trials.size(), conf._N_STAGE_TIMINGS > 0 ? conf._N_STAGE_TIMINGS : 0);
+        for (size_t i = 0; i < rows; ++i) {
+            const ppa_stats &x = trials[i];
+            csv << i + 1 << "," << x.grid_blocks << "," << x.block_threads << "," << g.n * g.d << ","
+                << x.prefetch_misses << "," << (x.total_ms - x.kernel_ms) * 1e-3 << "," << x.kernel_ms * 1e-3
+                << "\n";
+        }
     }
     std::printf("Computation Time: %f\n\n\n", st.total_ms * 1e-3);
     if (conf._FLAG_SHOW_DEGREE_DISTRIBUTION && (rc = printDegreeDistribution(conf)) != PPA_OK) return rc;

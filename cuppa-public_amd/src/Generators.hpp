@@ -34,7 +34,7 @@ public:
     const int32_t *deviceTargets() const { return _dGraph; }
 
 private:
-    int run(const ExperimentConfiguration &conf, const char *timingCsv);
+    int run(const ExperimentConfiguration &conf, const char *timingCsv, bool adaptive);
     ppa_graph_desc desc(const ExperimentConfiguration &conf) const;
 
     int64_t _N;

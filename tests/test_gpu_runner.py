@@ -61,3 +61,22 @@ def test_runner_gpu_csr_files(tmp_path, oracle_mod):
     assert (n2, nnz, kind) == (n, n * d, 1)
     assert np.array_equal(ptr, want_ptr)
     assert np.array_equal(idx, want_src)
+
+
+def test_runner_adaptive_trials(tmp_path, oracle_mod):
+    """`-A cuppa-adaptive` tunes the grid on the graph itself: one CSV row per
+    trial launch (the full grid, then 1/8, 1/4, ... of it), and the graph left
+    in the buffer is the oracle's whatever the trial."""
+    n, d, p, seed = 2_000_000, 8, 0.2, 21
+    r = subprocess.run([RUNNER, "-A", "cuppa-adaptive", "-n", str(n), "-x", str(d), "-a", str(p), "--seed",
+                        str(seed), "--show-time-n-stages", "8", "--edges-file", str(tmp_path / "g.edges")],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = (tmp_path / "StepwiseTiming-Adaptive.csv").read_text().splitlines()
+    rows = [l.split(",") for l in lines[1:]]
+    assert 2 <= len(rows) <= 4
+    assert all(row[3] == str(n * d) for row in rows)
+    assert int(rows[1][1]) < int(rows[0][1])  # the second trial is a fraction of the full grid
+    G, _, _ = oracle_mod.generate(n, d, oracle_mod.p_threshold(p), seed)
+    edges = np.loadtxt(tmp_path / "g.edges", dtype=np.int64)
+    assert np.array_equal(edges[d * (d + 1) // 2:, 1], G[(d + 1) * d:])
