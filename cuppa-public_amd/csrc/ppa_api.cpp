@@ -329,4 +329,32 @@ int ppa_in_csr(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_target
     return PPA_OK;
 }
 
+int ppa_undirected_csr(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_targets, int64_t *d_ptr,
+                       int32_t *d_adj, void *stream_)
+{
+    if (!ctx || !d_targets || !d_ptr || !d_adj) return PPA_ERR_INVALID;
+    int rc = ppa_check_desc(desc);
+    if (rc) return rc;
+    DeviceGuard g(ctx->device);
+    if (!g.ok()) return PPA_ERR_DEVICE;
+    hipStream_t stream = (hipStream_t)stream_;
+    const uint64_t n = desc->n, d = desc->d;
+    // the in-edge CSR goes to the back of the work area, after the sort's buffers
+    const size_t sort_bytes = (csr_work_bytes(n, (uint32_t)d, (uint32_t)ctx->num_cus) + 255) / 256 * 256;
+    const size_t ptr_bytes = ((n + 1) * 8 + 255) / 256 * 256;
+    if ((rc = ensure_work(ctx, sort_bytes + ptr_bytes + n * d * 4, stream)) != PPA_OK) return rc;
+    int64_t *in_ptr = (int64_t *)((char *)ctx->work + sort_bytes);
+    int32_t *in_src = (int32_t *)((char *)ctx->work + sort_bytes + ptr_bytes);
+    const int32_t *keys = nullptr;
+    PPA_HIP(launch_in_csr(d_targets, n, (uint32_t)d, in_ptr, in_src, ctx->work, (uint32_t)ctx->num_cus, stream,
+                          &keys));
+    int64_t total = 0;
+    PPA_HIP(hipMemcpyAsync(&total, in_ptr + n, sizeof(total), hipMemcpyDeviceToHost, stream));
+    PPA_HIP(hipStreamSynchronize(stream));
+    if ((uint64_t)total != n * d) return PPA_ERR_INVALID;  // an incomplete graph has no undirected form
+    PPA_HIP(launch_undirected(d_targets, n, (uint32_t)d, in_ptr, keys, in_src, d_ptr, d_adj, stream));
+    PPA_HIP(hipStreamSynchronize(stream));
+    return PPA_OK;
+}
+
 }  // extern "C"

@@ -570,6 +570,37 @@ __global__ void csr_ptr_kernel(const int32_t *keys, const uint64_t *total_p, uin
     }
 }
 
+// Undirected lists from the out rows and the in-edge CSR: node v's list is
+// its d targets (slot order), then the sources t >= d+1 of the slots naming it
+// (ascending); ptr[v] = v*d + in_ptr[v] - d*min(v, d+1). The in-edge sort's
+// position i of (v, t) maps to list index i + d*(v - d) for v > d; a seed node
+// v <= d keeps position i, and its d seed-row sources (t <= d), which sort
+// first, give their places to its own row.
+__global__ void undirected_in_kernel(const int32_t *keys, const int32_t *src, uint64_t total, uint32_t d,
+                                     int32_t *adj)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int32_t t = src[i];
+        if ((uint32_t)t <= d) continue;
+        const uint64_t v = (uint32_t)keys[i];
+        adj[i + (v > d ? (v - d) * d : 0)] = t;
+    }
+}
+
+__global__ void undirected_out_kernel(const int32_t *G, const int64_t *in_ptr, uint64_t n, uint32_t d, int64_t *ptr,
+                                      int32_t *adj)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= n; v += stride) {
+        const uint64_t seeds = v < (uint64_t)d + 1 ? v : (uint64_t)d + 1;
+        const int64_t base = (int64_t)(v * d) + in_ptr[v] - (int64_t)(seeds * d);
+        ptr[v] = base;
+        if (v < n)
+            for (uint32_t k = 0; k < d; ++k) adj[base + k] = G[v * d + k];
+    }
+}
+
 struct CsrGeometry {
     uint64_t slots = 0;
     uint32_t passes = 1, blocks = 1;
@@ -679,7 +710,7 @@ size_t csr_work_bytes(uint64_t n, uint32_t d, uint32_t num_cus)
 }
 
 hipError_t launch_in_csr(const int32_t *G, uint64_t n, uint32_t d, int64_t *in_ptr, int32_t *in_src, void *work,
-                         uint32_t num_cus, hipStream_t stream)
+                         uint32_t num_cus, hipStream_t stream, const int32_t **sorted_keys)
 {
     const CsrGeometry g = csr_geometry(n, d, num_cus);
     char *p = (char *)work;
@@ -720,6 +751,17 @@ hipError_t launch_in_csr(const int32_t *G, uint64_t n, uint32_t d, int64_t *in_p
     }
     hipLaunchKernelGGL(csr_ptr_kernel, dim3(grid_of(g.slots + 1, kThreads, 16384)), dim3(kThreads), 0, stream, kin,
                        (const uint64_t *)(seg_start + 1), n, g.slots, in_ptr);
+    if (sorted_keys) *sorted_keys = kin;
+    return hipGetLastError();
+}
+
+hipError_t launch_undirected(const int32_t *G, uint64_t n, uint32_t d, const int64_t *in_ptr, const int32_t *keys,
+                             const int32_t *in_src, int64_t *ptr, int32_t *adj, hipStream_t stream)
+{
+    hipLaunchKernelGGL(undirected_in_kernel, dim3(grid_of(n * d, kThreads, 16384)), dim3(kThreads), 0, stream, keys,
+                       in_src, n * d, d, adj);
+    hipLaunchKernelGGL(undirected_out_kernel, dim3(grid_of(n + 1, kThreads, 16384)), dim3(kThreads), 0, stream, G,
+                       in_ptr, n, d, ptr, adj);
     return hipGetLastError();
 }
 

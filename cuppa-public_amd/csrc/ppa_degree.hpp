@@ -35,7 +35,14 @@ hipError_t launch_degree_partitioned(const int32_t *G, uint64_t n, uint32_t d, u
 // in_ptr[v+1]) are the sources of the slots naming v in ascending order;
 // in_ptr[n] = valid slots. work holds csr_work_bytes(n, d, num_cus) bytes.
 size_t csr_work_bytes(uint64_t n, uint32_t d, uint32_t num_cus);
+// sorted_keys (nullable) receives the work-area buffer holding the targets in
+// in_src order, valid until the work area is reused.
 hipError_t launch_in_csr(const int32_t *G, uint64_t n, uint32_t d, int64_t *in_ptr, int32_t *in_src, void *work,
-                         uint32_t num_cus, hipStream_t stream);
+                         uint32_t num_cus, hipStream_t stream, const int32_t **sorted_keys = nullptr);
+// Undirected lists (out row, then the in-list without seed-row sources) of a
+// complete graph, given its in-edge CSR and the sorted targets of launch_in_csr.
+// ptr int64[n+1], adj int32[ptr[n]].
+hipError_t launch_undirected(const int32_t *G, uint64_t n, uint32_t d, const int64_t *in_ptr, const int32_t *keys,
+                             const int32_t *in_src, int64_t *ptr, int32_t *adj, hipStream_t stream);
 
 }  // namespace ppa

@@ -105,6 +105,7 @@ _SIGNATURES = [
     ("ppa_degree_histogram", ctypes.c_int,
      [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP, ctypes.c_uint64, _VP, _VP]),
     ("ppa_in_csr", ctypes.c_int, [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP, _VP]),
+    ("ppa_undirected_csr", ctypes.c_int, [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP, _VP]),
     ("ppa_batch_create", ctypes.c_int,
      [_VP, ctypes.POINTER(GraphDesc), ctypes.c_uint64, _VP, _VP, ctypes.POINTER(_VP)]),
     ("ppa_batch_launch", ctypes.c_int, [_VP, _VP]),
@@ -263,6 +264,18 @@ class Context:
     def in_csr(self, g: Graph, stream=None):
         """In-edge CSR (ppa_in_csr): (in_ptr int64 [n+1], in_src int32 [in_ptr[n]])."""
         return _in_csr(self, g, stream)
+
+    def undirected_csr(self, g: Graph, stream=None):
+        """Undirected lists (ppa_undirected_csr): (ptr int64 [n+1], adj int32 [ptr[n]])."""
+        import torch
+        dev = torch.device("cuda", self.device)
+        desc = make_desc(g.n, g.d, p_thresh=0)
+        ptr = torch.empty(g.n + 1, dtype=torch.int64, device=dev)
+        adj = torch.empty(max(1, 2 * g.n * g.d - (g.d + 1) * g.d), dtype=torch.int32, device=dev)
+        _check(lib().ppa_undirected_csr(self._h, ctypes.byref(desc), ctypes.c_void_p(g.targets.data_ptr()),
+                                        ctypes.c_void_p(ptr.data_ptr()), ctypes.c_void_p(adj.data_ptr()),
+                                        _stream_handle(stream)), "ppa_undirected_csr")
+        return ptr, adj
 
     def degree_histogram(self, g: Graph, hist_len: int | None = None, want_degree: bool = False, stream=None):
         """Returns (hist uint64 tensor, max_degree int, degree tensor or None)."""

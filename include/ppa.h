@@ -153,6 +153,14 @@ int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t
  * stream. */
 int ppa_in_csr(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_targets, int64_t *d_in_ptr,
                int32_t *d_in_src, void *stream);
+/* Undirected lists of a complete graph: d_adj[d_ptr[v] .. d_ptr[v+1]) = v's d
+ * targets (slot order), then the sources t >= d+1 of the slots naming v
+ * (ascending), so each seed-clique edge appears once per endpoint and
+ * d_ptr[v+1] - d_ptr[v] = deg(v) of ppa_degree_histogram. d_ptr int64[n+1],
+ * d_adj int32[2*n*d - (d+1)*d]. Work memory: that of ppa_in_csr plus
+ * 8*(n+1) + 4*n*d bytes. PPA_ERR_INVALID if a slot is out of range. */
+int ppa_undirected_csr(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_targets, int64_t *d_ptr,
+                       int32_t *d_adj, void *stream);
 
 /* ---- batched ensemble (no counterpart in the reference: it generates one
  *      graph per process run; this is cuPPA::generateGraph over many graphs

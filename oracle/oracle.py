@@ -68,6 +68,8 @@ def lib():
     L.oracle_degrees.restype = ctypes.c_int
     L.oracle_in_csr.argtypes = [u64, u32, i32p, ctypes.POINTER(ctypes.c_int64), i32p]
     L.oracle_in_csr.restype = ctypes.c_int
+    L.oracle_undirected_csr.argtypes = [u64, u32, i32p, ctypes.POINTER(ctypes.c_int64), i32p]
+    L.oracle_undirected_csr.restype = ctypes.c_int
     L.oracle_verify_rows.argtypes = [u64, u32, u64, u64, i32p, u64, u64, ctypes.POINTER(u64)]
     L.oracle_verify_rows.restype = ctypes.c_int
     L.oracle_verify_parallel.argtypes = [u64, u32, u64, u64, i32p, u64, u64, ctypes.c_int,
@@ -203,6 +205,18 @@ def in_csr(n: int, d: int, G: np.ndarray):
     if rc != 0:
         raise RuntimeError(f"oracle_in_csr failed: {rc}")
     return in_ptr, in_src[:in_ptr[n]]
+
+
+def undirected_csr(n: int, d: int, G: np.ndarray):
+    """Returns (ptr int64 [n+1], adj int32 [ptr[n]]) of a complete graph."""
+    G = np.ascontiguousarray(G, dtype=np.int32)
+    ptr = np.empty(n + 1, dtype=np.int64)
+    adj = np.empty(max(1, 2 * n * d - (d + 1) * d), dtype=np.int32)
+    rc = lib().oracle_undirected_csr(n, d, _ptr(G, ctypes.c_int32), _ptr(ptr, ctypes.c_int64),
+                                     _ptr(adj, ctypes.c_int32))
+    if rc != 0:
+        raise RuntimeError(f"oracle_undirected_csr failed: {rc}")
+    return ptr, adj[:ptr[n]]
 
 
 # ---------------------------------------------------------------------------

@@ -349,6 +349,35 @@ int oracle_in_csr(uint64_t n, uint32_t d, const int32_t *G, int64_t *in_ptr, int
     return ORACLE_OK;
 }
 
+/*
+ * Undirected lists: node v's d targets (slot order), then the sources t >= d+1
+ * of the slots naming v, ascending. Seed-row sources are left out, so each
+ * seed-clique edge appears once per endpoint and |list(v)| = deg(v) of
+ * oracle_degrees. ptr int64[n+1]; adj int32[2*n*d - (d+1)*d]. G complete.
+ */
+int oracle_undirected_csr(uint64_t n, uint32_t d, const int32_t *G, int64_t *ptr, int32_t *adj)
+{
+    int64_t *in_ptr = (int64_t *)malloc((n + 1) * sizeof(int64_t));
+    int32_t *in_src = (int32_t *)malloc(n * (uint64_t)d * sizeof(int32_t));
+    if (!in_ptr || !in_src) {
+        free(in_ptr);
+        free(in_src);
+        return ORACLE_ERR_NOMEM;
+    }
+    int rc = oracle_in_csr(n, d, G, in_ptr, in_src);
+    int64_t o = 0;
+    for (uint64_t v = 0; rc == ORACLE_OK && v < n; ++v) {
+        ptr[v] = o;
+        for (uint32_t k = 0; k < d; ++k) adj[o++] = G[v * d + k];
+        for (int64_t i = in_ptr[v]; i < in_ptr[v + 1]; ++i)
+            if ((uint32_t)in_src[i] > d) adj[o++] = in_src[i];
+    }
+    ptr[n] = o;
+    free(in_ptr);
+    free(in_src);
+    return rc;
+}
+
 /* ---- row-consistency verifier --------------------------------------------- */
 /*
  * Checks rows [first, first + count) of a complete candidate graph G: each row

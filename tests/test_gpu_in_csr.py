@@ -37,3 +37,23 @@ def test_in_csr_skips_out_of_range(ctx, oracle_mod):
     ptr, src = ctx.in_csr(g)
     assert np.array_equal(ptr.cpu().numpy(), want_ptr)
     assert np.array_equal(src.cpu().numpy(), want_src)
+
+
+@pytest.mark.parametrize("n,d,p,seed", [(300, 4, 0.5, 1), (70_000, 16, 0.1, 2), (2_500_000, 3, 0.3, 3)])
+def test_undirected_matches_oracle(ctx, oracle_mod, n, d, p, seed):
+    g = ctx.generate(n, d, p_thresh=oracle_mod.p_threshold(p), seed=seed)
+    G = g.targets.cpu().numpy()
+    want_ptr, want_adj = oracle_mod.undirected_csr(n, d, G)
+    ptr, adj = ctx.undirected_csr(g)
+    assert np.array_equal(ptr.cpu().numpy(), want_ptr)
+    assert np.array_equal(adj.cpu().numpy(), want_adj)
+
+
+def test_undirected_rejects_incomplete(ctx):
+    import torch
+    import cuppa
+    n, d = 1000, 2
+    G = np.zeros(n * d, np.int32)
+    G[5] = -1
+    with pytest.raises(cuppa.PPAError):
+        ctx.undirected_csr(cuppa.Graph(n, d, torch.from_numpy(G).cuda(), None))

@@ -246,3 +246,22 @@ def test_in_csr_oracle_pinned(oracle_mod):
     assert ptr2[n] == n * d - 3
     keep = (H >= 0) & (H < n)
     assert np.array_equal(src2, (np.argsort(np.where(keep, H, n), kind="stable")[: ptr2[n]] // d).astype(np.int32))
+
+
+def test_undirected_oracle_pinned(oracle_mod):
+    """Undirected lists: symmetric as multisets (v in list(u) as often as u in
+    list(v)), |list(v)| = deg(v) of the degree pass, each list starting with
+    the node's own row."""
+    n, d = 4000, 5
+    G, _, _ = oracle_mod.generate(n, d, oracle_mod.p_threshold(0.3), 9)
+    ptr, adj = oracle_mod.undirected_csr(n, d, G)
+    deg, _, _ = oracle_mod.degrees(n, d, G)
+    assert np.array_equal(np.diff(ptr), deg.astype(np.int64))
+    assert ptr[n] == 2 * n * d - (d + 1) * d
+    src = np.repeat(np.arange(n), np.diff(ptr))
+    pairs = np.stack([src, adj.astype(np.int64)], 1)
+    fwd = np.unique(pairs, axis=0, return_counts=True)
+    rev = np.unique(pairs[:, ::-1], axis=0, return_counts=True)
+    assert np.array_equal(fwd[0], rev[0]) and np.array_equal(fwd[1], rev[1])
+    for v in (0, d, d + 1, n - 1):
+        assert np.array_equal(adj[ptr[v]:ptr[v] + d], G[v * d:(v + 1) * d])
