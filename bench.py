@@ -298,6 +298,9 @@ def main():
                     help="nodes of the CPU baseline sample (0 = skip)")
     ap.add_argument("--degree", action="store_true", help="also time the degree-histogram pass")
     ap.add_argument("--in-csr", action="store_true", help="also time the in-edge CSR build")
+    ap.add_argument("--ba", action="store_true",
+                    help="also time the Barabasi-Albert model (-A sba) at the workload's n and d, "
+                         "and the reference's serial BA on a prefix")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB three times (counter calibration) and exit")
     ap.add_argument("--mode", default="graph", choices=("graph", "batch", "serve"),
@@ -406,6 +409,33 @@ def main():
         extra["in_csr_ms"] = times[1]
         extra["in_csr_first_call_ms"] = times[0]
         extra["in_csr_edges_per_s"] = edges / (times[1] * 1e-3)
+    if args.ba:
+        # same n, d and buffers; the first call is the warm-up
+        ba_desc = cuppa.make_desc(n, d, seed=seed, model=cuppa.MODEL_BA)
+        ctx.generate_into(ba_desc, targets, row_ptr, stream)
+        ba_k, ba_wall = [], []
+        for _ in range(max(1, args.steps)):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            ctx.generate_into(ba_desc, targets, row_ptr, stream)
+            ba_wall.append(time.perf_counter() - t1)
+            ba_k.append(ctx.stats()["kernel_ms"])
+        bk = sum(ba_k) / len(ba_k)
+        bw = sum(ba_wall) / len(ba_wall)
+        ba = {"edges_per_s": edges / bw, "ms_per_call": bw * 1e3, "kernel_ms": bk,
+              "roofline_frac": algorithmic_bytes_per_slot(0.5, d)["generate_kernel"] * edges / (bk * 1e-3) / 1e9
+              / HBM_PEAK_GBS}
+        ref_ba = os.path.join(ROOT, "oracle", "_ref", "sba_ref")
+        if args.cpu_prefix and os.path.exists(ref_ba):
+            with tempfile.TemporaryDirectory() as tmp:
+                cmd = [ref_ba, str(args.cpu_prefix), str(d), tmp + "/", "0"]
+                cmd = ["taskset", "-c", "0"] + cmd if _has("taskset") else cmd
+                out = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=300).stdout
+                secs = float(re.search(r"Computation Time\s*:\s*([0-9.eE+-]+)", out).group(1))
+            ba["cpu_baseline"] = {"value": args.cpu_prefix * d / secs, "unit": "edges/s", "cores": 1,
+                                  "kind": "reference", "seconds": secs,
+                                  "sample": f"reference SerialBA (MT19937-64), first {args.cpu_prefix} nodes, d={d}"}
+        extra["ba"] = ba
     copy_gbs = copy_bandwidth_gbs(torch)
 
     del targets, row_ptr

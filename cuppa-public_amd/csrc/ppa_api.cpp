@@ -141,6 +141,7 @@ int ppa_check_desc(const ppa_graph_desc *desc)
     if (desc->d < 1 || desc->d > PPA_MAX_D) return PPA_ERR_INVALID;
     if (desc->n < (uint64_t)desc->d + 1 || desc->n > PPA_MAX_N) return PPA_ERR_INVALID;
     if (desc->p_threshold > (1ull << 32)) return PPA_ERR_INVALID;
+    if (desc->model != PPA_MODEL_COPY && desc->model != PPA_MODEL_BA) return PPA_ERR_INVALID;
     return PPA_OK;
 }
 
@@ -230,14 +231,14 @@ int ppa_generate_async(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targ
     ctx->last.block_threads = shape.threads;
     PPA_HIP(hipEventRecord(ctx->ev[0], stream));
     PPA_HIP(launch_reset_scratch(ctx->scratch, (uint64_t)d + 1, stream));
-    FillArgs fa{d_targets, d_row_ptr, n, d, 0};
+    FillArgs fa{d_targets, d_row_ptr, n, d, desc->model};
     PPA_HIP(launch_fill(fa, stream));
     PPA_HIP(hipEventRecord(ctx->ev[1], stream));
     if (n > (uint64_t)d + 1) {
         GenArgs ga{};
         ga.G = d_targets;
         ga.n = n;
-        ga.T = desc->p_threshold;
+        ga.T = coin_threshold(desc->model, desc->p_threshold);
         ga.key0 = (uint32_t)desc->seed;
         ga.key1 = (uint32_t)(desc->seed >> 32);
         ga.chunk = shape.chunk;
@@ -304,6 +305,7 @@ int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t
     } else {
         PPA_HIP(launch_degree_atomic(d_targets, desc->n, desc->d, deg, stream));
     }
+    PPA_HIP(launch_seed_degree(deg, desc->d, desc->model, stream));
     PPA_HIP(launch_degree_histogram(deg, desc->n, d_hist, hist_len, ctx->scratch, stream));
     if (d_max_degree)
         PPA_HIP(hipMemcpyAsync(d_max_degree, &ctx->scratch->max_degree, sizeof(uint64_t),
@@ -337,6 +339,7 @@ int ppa_undirected_csr(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *
     if (rc) return rc;
     DeviceGuard g(ctx->device);
     if (!g.ok()) return PPA_ERR_DEVICE;
+    if (desc->model != PPA_MODEL_COPY) return PPA_ERR_INVALID;
     hipStream_t stream = (hipStream_t)stream_;
     const uint64_t n = desc->n, d = desc->d;
     // the in-edge CSR goes to the back of the work area, after the sort's buffers

@@ -45,10 +45,11 @@ BatchGraph graph_record(const ppa_graph_desc &dsc)
     BatchGraph gr;
     std::memset(&gr, 0, sizeof(gr));
     gr.n = dsc.n;
-    gr.T = dsc.p_threshold;
+    gr.T = coin_threshold(dsc.model, dsc.p_threshold);
     gr.key0 = (uint32_t)dsc.seed;
     gr.key1 = (uint32_t)(dsc.seed >> 32);
     gr.d = dsc.d;
+    gr.model = dsc.model;
     gr.inv_d = 1.0 / (double)dsc.d;
     return gr;
 }
@@ -306,7 +307,7 @@ int ppa_autotune(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, int3
     if (count == 1) {
         const ppa_graph_desc &g = descs[0];
         const int nb = (int)std::floor(std::log2((double)g.n));
-        const int pb = (int)std::lround((double)g.p_threshold / 4294967296.0 * 10.0);
+        const int pb = (int)std::lround((double)coin_threshold(g.model, g.p_threshold) / 4294967296.0 * 10.0);
         sig = "g:" + std::to_string(nb) + ":" + std::to_string(g.d) + ":" + std::to_string(pb);
     } else {
         double work = 0, dsum = 0;

@@ -1,4 +1,5 @@
-// ppa_cpu.cpp -- the serial CPU copy model of the product (`-A scm`).
+// ppa_cpu.cpp -- the serial CPU copy model of the product (`-A scm`); also
+// runs the Barabasi-Albert model (PPA_MODEL_BA) through the same loop.
 //
 // Counterpart of SerialCopyModel::generateGraphCopyModelBST
 // (src/PA/SPA/SerialCopyModel3.cpp:11-107): one host thread walks the nodes in
@@ -20,10 +21,10 @@ int ppa_cpu_generate(const ppa_graph_desc *desc, int32_t *targets, int64_t *row_
     if (!targets) return PPA_ERR_INVALID;
     const uint64_t n = desc->n;
     const uint32_t d = desc->d;
-    const uint64_t T = desc->p_threshold;
+    const uint64_t T = ppa::coin_threshold(desc->model, desc->p_threshold);
     const uint32_t k0 = (uint32_t)desc->seed, k1 = (uint32_t)(desc->seed >> 32);
     for (uint64_t i = 0; i <= d; ++i)
-        for (uint32_t j = 0; j < d; ++j) targets[i * d + j] = ppa::clique_target(i, j);
+        for (uint32_t j = 0; j < d; ++j) targets[i * d + j] = ppa::seed_target(desc->model, d, i, j);
     uint64_t retries = 0;
     for (uint64_t t = (uint64_t)d + 1; t < n; ++t) {
         int32_t *row = targets + t * d;
@@ -57,7 +58,7 @@ int ppa_cpu_degree_histogram(const ppa_graph_desc *desc, const int32_t *targets,
     if (!targets || !degree || (hist_len && !hist)) return PPA_ERR_INVALID;
     const uint64_t n = desc->n;
     const uint32_t d = desc->d;
-    for (uint64_t v = 0; v < n; ++v) degree[v] = d;
+    for (uint64_t v = 0; v < n; ++v) degree[v] = ppa::base_degree(desc->model, d, v);
     for (uint64_t s = (uint64_t)(d + 1) * d; s < n * d; ++s) {
         const int32_t v = targets[s];
         if (v < 0 || (uint64_t)v >= n) return PPA_ERR_INVALID;

@@ -9,6 +9,7 @@
 
 #include "../../include/ppa.h"
 #include "ppa_kernels.hpp"
+#include "ppa_model.hpp"
 
 struct ppa_ctx {
     int device = 0;

@@ -111,7 +111,7 @@ __global__ void reset_scratch_kernel(DeviceScratch *s, uint64_t first)
     if (threadIdx.x < kStatCount) s->stats[threadIdx.x] = 0;
 }
 
-// Sets generated slots to -1, writes the seed clique rows and row_ptr[t] = t*d.
+// Sets generated slots to -1, writes the seed rows 0..d and row_ptr[t] = t*d.
 __global__ void fill_kernel(FillArgs a)
 {
     const uint64_t slots = a.n * a.d;
@@ -129,14 +129,14 @@ __global__ void fill_kernel(FillArgs a)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const uint64_t s = e + i;
-                w[i] = s < clique ? clique_target(s / a.d, (uint32_t)(s % a.d)) : kEmpty;
+                w[i] = s < clique ? seed_target(a.model, a.d, s / a.d, (uint32_t)(s % a.d)) : kEmpty;
             }
             v = make_int4(w[0], w[1], w[2], w[3]);
         }
         G4[g] = v;
     }
     for (uint64_t s = groups * 4 + tid; s < slots; s += stride)
-        a.G[s] = s < clique ? clique_target(s / a.d, (uint32_t)(s % a.d)) : kEmpty;
+        a.G[s] = s < clique ? seed_target(a.model, a.d, s / a.d, (uint32_t)(s % a.d)) : kEmpty;
     if (a.row_ptr)
         for (uint64_t t = tid; t <= a.n; t += stride) a.row_ptr[t] = (int64_t)(t * a.d);
 }
@@ -174,7 +174,8 @@ __global__ __launch_bounds__(256) void fill_batch_kernel(BatchFillArgs a, uint64
             const BatchGraph &gr = a.graphs[g];
             const uint64_t local = s - a.slot_base[g];
             const uint64_t clique = (uint64_t)(gr.d + 1) * gr.d;
-            gr.G[local] = local < clique ? clique_target(local / gr.d, (uint32_t)(local % gr.d)) : kEmpty;
+            gr.G[local] =
+                local < clique ? seed_target(gr.model, gr.d, local / gr.d, (uint32_t)(local % gr.d)) : kEmpty;
         }
     }
     for (uint64_t base = blockIdx.x * per_block; base < total_rows; base += gridDim.x * per_block) {
@@ -714,6 +715,14 @@ __global__ void degree_count_kernel(const int32_t *G, uint64_t first_slot, uint6
     }
 }
 
+// deg[v] += base_degree(model, d, v) - d for the seed nodes v < d: the
+// Barabasi-Albert count (PAHelper.hpp:84-121) leaves out their self-loops.
+__global__ void seed_degree_kernel(uint32_t *deg, uint32_t d, uint32_t model)
+{
+    const uint32_t v = threadIdx.x;
+    if (v < d) deg[v] = deg[v] - d + base_degree(model, d, v);
+}
+
 constexpr int kHistLds = 4096;
 
 // hist[deg[v]]++ (cuPPA.cu:147-158); small degrees are counted in LDS first.
@@ -838,6 +847,13 @@ hipError_t launch_degree_atomic(const int32_t *G, uint64_t n, uint32_t d, uint32
     if (slots > first)
         hipLaunchKernelGGL(degree_count_kernel, dim3(grid_for(slots - first, 256, 16384)), dim3(256), 0,
                            stream, G, first, slots, n, deg);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_degree(uint32_t *deg, uint32_t d, uint32_t model, hipStream_t stream)
+{
+    if (model != PPA_MODEL_COPY)
+        hipLaunchKernelGGL(seed_degree_kernel, dim3(1), dim3(64), 0, stream, deg, d, model);
     return hipGetLastError();
 }
 

@@ -38,7 +38,7 @@ struct BatchGraph {
     uint64_t targets_offset;  // added to every row_ptr entry
     uint32_t key0, key1;      // Philox key
     uint32_t d;
-    uint32_t pad;
+    uint32_t model;           // PPA_MODEL_* (seed rows)
     double inv_d;             // 1.0 / d
 };
 static_assert(sizeof(BatchGraph) == 64, "BatchGraph layout");
@@ -81,7 +81,7 @@ struct FillArgs {
     int64_t *row_ptr;  // nullable
     uint64_t n;
     uint32_t d;
-    uint32_t pad;
+    uint32_t model;    // PPA_MODEL_*: which seed rows
 };
 
 struct BatchFillArgs {
@@ -101,6 +101,8 @@ hipError_t launch_generate(uint32_t d, const GenArgs &a, uint32_t blocks, uint32
 hipError_t launch_generate_batch(const GenArgs &a, uint32_t blocks, uint32_t block_threads, hipStream_t stream);
 hipError_t generate_occupancy(uint32_t d, uint32_t block_threads, int *blocks_per_cu);
 hipError_t launch_degree_atomic(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, hipStream_t stream);
+// Adjusts the seed nodes' degrees of a non-copy model (d <= 64 threads).
+hipError_t launch_seed_degree(uint32_t *deg, uint32_t d, uint32_t model, hipStream_t stream);
 hipError_t launch_degree_histogram(const uint32_t *deg, uint64_t n, uint64_t *hist, uint64_t hist_len,
                                    DeviceScratch *s, hipStream_t stream);
 

@@ -13,6 +13,8 @@
 
 #include <stdint.h>
 
+#include "ppa.h"
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define PPA_HD __host__ __device__ __forceinline__
@@ -87,5 +89,28 @@ PPA_HD Attempt attempt(uint32_t key0, uint32_t key1, uint64_t T, uint64_t t, uin
 
 // Seed clique on nodes 0..d: node i's slot j names j + (j >= i).
 PPA_HD int32_t clique_target(uint64_t i, uint32_t j) { return (int32_t)(j + (j >= i ? 1u : 0u)); }
+
+// Seed rows 0..d of a graph of the given model. Barabasi-Albert seed nodes
+// i < d hold self-loops (SerialBA.cpp:90-97); its node d names 0..d-1
+// (SerialBA.cpp:108-112), which is the clique's row d.
+PPA_HD int32_t seed_target(uint32_t model, uint32_t d, uint64_t i, uint32_t j)
+{
+    return model == PPA_MODEL_BA && i < d ? (int32_t)i : clique_target(i, j);
+}
+
+// Coin threshold the generator uses: a Barabasi-Albert endpoint is the source
+// of its (node, slot) pair with probability exactly 1/2.
+PPA_HD uint64_t coin_threshold(uint32_t model, uint64_t p_threshold)
+{
+    return model == PPA_MODEL_BA ? (1ull << 31) : p_threshold;
+}
+
+// Degree every node starts from before the slots of nodes t >= d+1 are
+// counted: its own d slots (copy model), or, for Barabasi-Albert, d for
+// v >= d and the one slot of node d for a seed node (self-loops uncounted).
+PPA_HD uint32_t base_degree(uint32_t model, uint32_t d, uint64_t v)
+{
+    return model == PPA_MODEL_BA && v < d ? 1u : d;
+}
 
 }  // namespace ppa

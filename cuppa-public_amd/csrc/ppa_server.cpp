@@ -154,6 +154,7 @@ ppa_graph_desc desc_of(const ppa_request &r)
     ppa_graph_desc g{};
     g.n = r.n;
     g.d = r.d;
+    g.model = r.model;
     g.p_threshold = r.p_threshold;
     g.seed = r.seed;
     return g;
@@ -192,13 +193,13 @@ int launch(ppa_server *s, Slot &sl, const std::vector<uint64_t> &ids)
     if (single) {
         const Request &r = s->reqs[ids[0]];
         PPA_HIP(launch_reset_scratch(sl.scratch, (uint64_t)r.req.d + 1, sl.stream));
-        FillArgs fa{graphs[0].G, graphs[0].row_ptr, r.req.n, r.req.d, 0};
+        FillArgs fa{graphs[0].G, graphs[0].row_ptr, r.req.n, r.req.d, r.req.model};
         PPA_HIP(launch_fill(fa, sl.stream));
         if (work) {
             GenArgs ga{};
             ga.G = graphs[0].G;
             ga.n = r.req.n;
-            ga.T = r.req.p_threshold;
+            ga.T = graphs[0].T;
             ga.key0 = (uint32_t)r.req.seed;
             ga.key1 = (uint32_t)(r.req.seed >> 32);
             ga.chunk = shape.chunk;

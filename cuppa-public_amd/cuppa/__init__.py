@@ -38,8 +38,11 @@ class PPAError(RuntimeError):
         super().__init__(f"{what}: {msg}" if what else msg)
 
 
+MODEL_COPY, MODEL_BA = 0, 1  # PPA_MODEL_*
+
+
 class GraphDesc(ctypes.Structure):
-    _fields_ = [("n", ctypes.c_uint64), ("d", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+    _fields_ = [("n", ctypes.c_uint64), ("d", ctypes.c_uint32), ("model", ctypes.c_uint32),
                 ("p_threshold", ctypes.c_uint64), ("seed", ctypes.c_uint64),
                 ("targets_offset", ctypes.c_uint64), ("row_ptr_offset", ctypes.c_uint64)]
 
@@ -74,7 +77,7 @@ class ServerConfig(ctypes.Structure):
 
 class Request(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint64), ("n", ctypes.c_uint64), ("d", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32), ("p_threshold", ctypes.c_uint64), ("seed", ctypes.c_uint64)]
+                ("model", ctypes.c_uint32), ("p_threshold", ctypes.c_uint64), ("seed", ctypes.c_uint64)]
 
 
 class Result(ctypes.Structure):
@@ -167,12 +170,13 @@ def p_threshold(p: float) -> int:
 
 
 def make_desc(n: int, d: int, p: float | None = None, seed: int = 0, p_thresh: int | None = None,
-              targets_offset: int = 0, row_ptr_offset: int = 0) -> GraphDesc:
+              targets_offset: int = 0, row_ptr_offset: int = 0, model: int = MODEL_COPY) -> GraphDesc:
+    """Descriptor of one graph; a Barabasi-Albert graph (model=MODEL_BA) needs no p."""
     if p_thresh is None:
-        if p is None:
+        if p is None and model != MODEL_BA:
             raise ValueError("give p or p_thresh")
-        p_thresh = p_threshold(p)
-    return GraphDesc(n, d, 0, p_thresh, seed & 0xFFFFFFFFFFFFFFFF, targets_offset, row_ptr_offset)
+        p_thresh = p_threshold(p) if p is not None else 0
+    return GraphDesc(n, d, model, p_thresh, seed & 0xFFFFFFFFFFFFFFFF, targets_offset, row_ptr_offset)
 
 
 def _check(rc: int, what: str):
@@ -193,6 +197,7 @@ class Graph:
     d: int
     targets: "object"  # torch.int32 [n*d] on the device
     row_ptr: "object"  # torch.int64 [n+1] on the device, or None
+    model: int = MODEL_COPY
 
 
 class Context:
@@ -251,15 +256,15 @@ class Context:
         _check(lib().ppa_ctx_sync(self._h, _stream_handle(stream)), "ppa_ctx_sync")
 
     def generate(self, n: int, d: int, p: float | None = None, seed: int = 0, p_thresh: int | None = None,
-                 row_ptr: bool = True, stream=None) -> Graph:
+                 row_ptr: bool = True, stream=None, model: int = MODEL_COPY) -> Graph:
         import torch
-        desc = make_desc(n, d, p, seed, p_thresh)
+        desc = make_desc(n, d, p, seed, p_thresh, model=model)
         _check(lib().ppa_check_desc(ctypes.byref(desc)), "ppa_check_desc")
         dev = torch.device("cuda", self.device)
         targets = torch.empty(n * d, dtype=torch.int32, device=dev)
         rp = torch.empty(n + 1, dtype=torch.int64, device=dev) if row_ptr else None
         self.generate_into(desc, targets, rp, stream)
-        return Graph(n, d, targets, rp)
+        return Graph(n, d, targets, rp, model)
 
     def in_csr(self, g: Graph, stream=None):
         """In-edge CSR (ppa_in_csr): (in_ptr int64 [n+1], in_src int32 [in_ptr[n]])."""
@@ -269,7 +274,7 @@ class Context:
         """Undirected lists (ppa_undirected_csr): (ptr int64 [n+1], adj int32 [ptr[n]])."""
         import torch
         dev = torch.device("cuda", self.device)
-        desc = make_desc(g.n, g.d, p_thresh=0)
+        desc = make_desc(g.n, g.d, p_thresh=0, model=g.model)
         ptr = torch.empty(g.n + 1, dtype=torch.int64, device=dev)
         adj = torch.empty(max(1, 2 * g.n * g.d - (g.d + 1) * g.d), dtype=torch.int32, device=dev)
         _check(lib().ppa_undirected_csr(self._h, ctypes.byref(desc), ctypes.c_void_p(g.targets.data_ptr()),
@@ -281,7 +286,7 @@ class Context:
         """Returns (hist uint64 tensor, max_degree int, degree tensor or None)."""
         import torch
         dev = torch.device("cuda", self.device)
-        desc = make_desc(g.n, g.d, p_thresh=0)
+        desc = make_desc(g.n, g.d, p_thresh=0, model=g.model)
         deg = torch.empty(g.n, dtype=torch.int32, device=dev) if want_degree else None
         mx = torch.zeros(1, dtype=torch.int64, device=dev)
         if hist_len is None:
@@ -303,7 +308,7 @@ class Context:
 def _in_csr(ctx, g: "Graph", stream=None):
     import torch
     dev = torch.device("cuda", ctx.device)
-    desc = make_desc(g.n, g.d, p_thresh=0)
+    desc = make_desc(g.n, g.d, p_thresh=0, model=g.model)
     in_ptr = torch.empty(g.n + 1, dtype=torch.int64, device=dev)
     in_src = torch.empty(g.n * g.d, dtype=torch.int32, device=dev)
     _check(lib().ppa_in_csr(ctx._h, ctypes.byref(desc), ctypes.c_void_p(g.targets.data_ptr()),
@@ -313,12 +318,15 @@ def _in_csr(ctx, g: "Graph", stream=None):
 
 
 def batch_layout(specs):
-    """specs: list of (n, d, p_threshold, seed). Returns (GraphDesc array,
-    total target slots, total row_ptr entries) with graphs laid out back to back."""
+    """specs: list of (n, d, p_threshold, seed[, model]). Returns (GraphDesc
+    array, total target slots, total row_ptr entries) with graphs laid out back
+    to back."""
     descs = (GraphDesc * len(specs))()
     toff = roff = 0
-    for i, (n, d, T, seed) in enumerate(specs):
-        descs[i] = GraphDesc(n, d, 0, T, seed & 0xFFFFFFFFFFFFFFFF, toff, roff)
+    for i, spec in enumerate(specs):
+        n, d, T, seed = spec[:4]
+        model = spec[4] if len(spec) > 4 else MODEL_COPY
+        descs[i] = GraphDesc(n, d, model, T, seed & 0xFFFFFFFFFFFFFFFF, toff, roff)
         toff += n * d
         roff += n + 1
     return descs, toff, roff
@@ -377,8 +385,8 @@ class Server:
         cfg = ServerConfig(budget_bytes, streams, max_batch, batch_nodes, solo_nodes, 1 if plain else 0, 0)
         _check(lib().ppa_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(self._h)), "ppa_server_create")
 
-    def submit(self, rid: int, n: int, d: int, p_thresh: int, seed: int) -> int:
-        return lib().ppa_server_submit(self._h, ctypes.byref(Request(rid, n, d, 0, p_thresh, seed)))
+    def submit(self, rid: int, n: int, d: int, p_thresh: int, seed: int, model: int = MODEL_COPY) -> int:
+        return lib().ppa_server_submit(self._h, ctypes.byref(Request(rid, n, d, model, p_thresh, seed)))
 
     def step(self) -> int:
         rc = lib().ppa_server_step(self._h)
@@ -403,10 +411,12 @@ class Server:
         return dict(zip(("used", "budget", "queued", "running"), (x.value for x in v)))
 
     def replay(self, specs, arrival_ms, with_wait=False):
-        """specs: list of (id, n, d, p_threshold, seed). Returns (latency_ms list,
-        makespan_ms), or (latency_ms, wait_ms, makespan_ms) with with_wait."""
+        """specs: list of (id, n, d, p_threshold, seed[, model]). Returns
+        (latency_ms list, makespan_ms), or (latency_ms, wait_ms, makespan_ms)
+        with with_wait."""
         count = len(specs)
-        reqs = (Request * count)(*[Request(i, n, d, 0, T, s) for (i, n, d, T, s) in specs])
+        reqs = (Request * count)(*[Request(sp[0], sp[1], sp[2], sp[5] if len(sp) > 5 else MODEL_COPY, sp[3],
+                                           sp[4]) for sp in specs])
         arr = (ctypes.c_double * count)(*arrival_ms)
         lat = (ctypes.c_double * count)()
         wait = (ctypes.c_double * count)()
@@ -472,6 +482,32 @@ class CuPPA:
     generate_graph_adaptive = generate_graph
 
     def degree_distribution(self) -> dict:
+        if self.graph is None:
+            raise RuntimeError("generate_graph() first")
+        hist, _, _ = self.ctx.degree_histogram(self.graph)
+        h = hist.cpu().tolist()
+        return {deg: c for deg, c in enumerate(h) if c > 0}
+
+
+class SerialBA:
+    """Mirror of the reference's ``SerialBA`` object (SerialBA.h:30-51):
+    ``SerialBA(n, m)`` is the Barabasi-Albert generator of ``-A sba`` with n
+    nodes and m edges per new node, run here on the GPU as a PPA_MODEL_BA
+    graph. ``seed`` keys the draws (the reference draws from MT19937-64 with
+    its default seed, Random64.hpp:75-76)."""
+
+    def __init__(self, n: int, m: int = 1, seed: int = 0, device: int = 0):
+        self._N, self._M, self.seed = n, m, seed
+        self.ctx = Context(device)
+        self.graph: Graph | None = None
+
+    def generate_graph(self) -> Graph:
+        self.graph = self.ctx.generate(self._N, self._M, seed=self.seed, model=MODEL_BA)
+        return self.graph
+
+    def degree_distribution(self) -> dict:
+        """Degree -> count as PAHelper::outputSBADegreeDistribution writes it
+        (PAHelper.hpp:69-147)."""
         if self.graph is None:
             raise RuntimeError("generate_graph() first")
         hist, _, _ = self.ctx.degree_histogram(self.graph)

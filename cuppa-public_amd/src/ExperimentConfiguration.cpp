@@ -100,7 +100,7 @@ void printUsage(const char *prog)
     std::printf(
         "usage: %s [options]\n"
         "  -c,--config-file FILE     INI file, section [config] (applied after the flags)\n"
-        "  -A,--algorithm NAME       cuppa | cuppa-adaptive | scm\n"
+        "  -A,--algorithm NAME       cuppa | cuppa-adaptive | scm | sba\n"
         "  -l,--loadbalancing NAME   accepted (rrp is the only scheme of the reference)\n"
         "  -n,--n1 N                 number of vertices\n"
         "  -x,--x1 M                 out-edges per new vertex (1..16)\n"

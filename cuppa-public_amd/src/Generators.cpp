@@ -26,15 +26,20 @@ bool writeDegreeCsv(const std::string &path, const std::vector<uint64_t> &hist)
     return bool(out);
 }
 
-bool writeEdgeList(const std::string &path, const int32_t *G, uint64_t n, uint32_t d)
+bool writeEdgeList(const std::string &path, const int32_t *G, uint64_t n, uint32_t d, uint32_t model)
 {
     FILE *f = std::fopen(path.c_str(), "w");
     if (!f) return false;
     std::vector<char> buf(1 << 20);
     std::setvbuf(f, buf.data(), _IOFBF, buf.size());
-    for (uint64_t i = 1; i <= d && i < n; ++i)
-        for (uint32_t j = 0; j < d; ++j)
-            if ((uint64_t)G[i * d + j] < i) std::fprintf(f, "%llu %d\n", (unsigned long long)i, G[i * d + j]);
+    if (model == PPA_MODEL_COPY) {
+        for (uint64_t i = 1; i <= d && i < n; ++i)
+            for (uint32_t j = 0; j < d; ++j)
+                if ((uint64_t)G[i * d + j] < i) std::fprintf(f, "%llu %d\n", (unsigned long long)i, G[i * d + j]);
+    } else {
+        for (uint32_t k = 0; k < d && d < n; ++k)
+            std::fprintf(f, "%llu %d\n", (unsigned long long)d, G[(uint64_t)d * d + k]);
+    }
     for (uint64_t t = (uint64_t)d + 1; t < n; ++t)
         for (uint32_t k = 0; k < d; ++k) std::fprintf(f, "%llu %d\n", (unsigned long long)t, G[t * d + k]);
     const bool ok = std::ferror(f) == 0;
@@ -104,9 +109,76 @@ CsrHeader csrHeader(const ppa_graph_desc &g, uint64_t nnz, uint32_t kind)
     h.d = g.d;
     h.kind = kind;
     h.seed = g.seed;
-    h.p_threshold = g.p_threshold;
+    h.p_threshold = g.model == PPA_MODEL_BA ? (1ull << 31) : g.p_threshold;
+    h.model = g.model;
     return h;
 }
+
+namespace {
+
+// Degree histogram of a generated graph on the device: ppa_degree_histogram
+// once for the maximum degree, then into a histogram of that length.
+// seedDeg (nullable) receives the degrees of nodes 0..d-1.
+int deviceDegreeHistogram(ppa_ctx *ctx, const ppa_graph_desc &g, const int32_t *dGraph,
+                          std::vector<uint64_t> *hist, std::vector<uint32_t> *seedDeg)
+{
+    uint32_t *dDeg = nullptr;
+    uint64_t *dMax = nullptr, *dHist = nullptr;
+    int rc = PPA_OK;
+    if (hipMalloc(&dDeg, g.n * sizeof(uint32_t)) != hipSuccess || hipMalloc(&dMax, sizeof(uint64_t)) != hipSuccess)
+        rc = PPA_ERR_NOMEM;
+    uint64_t mx = 0;
+    if (!rc) rc = ppa_degree_histogram(ctx, &g, dGraph, dDeg, nullptr, 0, dMax, nullptr);
+    if (!rc && hipMemcpy(&mx, dMax, sizeof(mx), hipMemcpyDeviceToHost) != hipSuccess) rc = PPA_ERR_DEVICE;
+    if (!rc) hist->assign(mx + 1, 0);
+    if (!rc && hipMalloc(&dHist, hist->size() * sizeof(uint64_t)) != hipSuccess) rc = PPA_ERR_NOMEM;
+    if (!rc) rc = ppa_degree_histogram(ctx, &g, dGraph, dDeg, dHist, hist->size(), dMax, nullptr);
+    if (!rc && hipMemcpy(hist->data(), dHist, hist->size() * sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = PPA_ERR_DEVICE;
+    if (!rc && seedDeg) {
+        seedDeg->assign(g.d, 0);
+        if (hipMemcpy(seedDeg->data(), dDeg, g.d * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = PPA_ERR_DEVICE;
+    }
+    if (dDeg) (void)hipFree(dDeg);
+    if (dMax) (void)hipFree(dMax);
+    if (dHist) (void)hipFree(dHist);
+    return rc;
+}
+
+// --edges-file, --csr-file and --in-csr-file of a graph held on the device.
+int writeDeviceGraphFiles(const ExperimentConfiguration &conf, ppa_ctx *ctx, const ppa_graph_desc &g,
+                          const int32_t *dGraph, const int64_t *dRowPtr)
+{
+    int rc = PPA_OK;
+    if (!conf._edgesFile.empty()) {
+        std::vector<int32_t> host(g.n * g.d);
+        if (hipMemcpy(host.data(), dGraph, host.size() * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+            return PPA_ERR_DEVICE;
+        if (!writeEdgeList(conf._edgesFile, host.data(), g.n, g.d, g.model)) return PPA_ERR_INVALID;
+    }
+    if (!conf._csrFile.empty() && !writeCsr(conf._csrFile, csrHeader(g, g.n * g.d, 0), dRowPtr, dGraph, true))
+        return PPA_ERR_INVALID;
+    if (!conf._inCsrFile.empty()) {
+        int64_t *dPtr = nullptr;
+        int32_t *dSrc = nullptr;
+        rc = hipMalloc(&dPtr, (g.n + 1) * sizeof(int64_t)) == hipSuccess &&
+                     hipMalloc(&dSrc, g.n * g.d * sizeof(int32_t)) == hipSuccess
+                 ? PPA_OK
+                 : PPA_ERR_NOMEM;
+        if (!rc) rc = ppa_in_csr(ctx, &g, dGraph, dPtr, dSrc, nullptr);
+        int64_t nnz = 0;
+        if (!rc && hipMemcpy(&nnz, dPtr + g.n, sizeof(nnz), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = PPA_ERR_DEVICE;
+        if (!rc && !writeCsr(conf._inCsrFile, csrHeader(g, (uint64_t)nnz, 1), dPtr, dSrc, true))
+            rc = PPA_ERR_INVALID;
+        if (dPtr) (void)hipFree(dPtr);
+        if (dSrc) (void)hipFree(dSrc);
+    }
+    return rc;
+}
+
+}  // namespace
 
 // ---- cuPPA ------------------------------------------------------------------
 
@@ -215,32 +287,7 @@ int cuPPA::run(const ExperimentConfiguration &conf, const char *timingCsv, bool 
     }
     std::printf("Computation Time: %f\n\n\n", st.total_ms * 1e-3);
     if (conf._FLAG_SHOW_DEGREE_DISTRIBUTION && (rc = printDegreeDistribution(conf)) != PPA_OK) return rc;
-    if (!conf._edgesFile.empty()) {
-        std::vector<int32_t> host(g.n * g.d);
-        if (hipMemcpy(host.data(), _dGraph, host.size() * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
-            return PPA_ERR_DEVICE;
-        if (!writeEdgeList(conf._edgesFile, host.data(), g.n, g.d)) return PPA_ERR_INVALID;
-    }
-    if (!conf._csrFile.empty() &&
-        !writeCsr(conf._csrFile, csrHeader(g, g.n * g.d, 0), _dRowPtr, _dGraph, true))
-        return PPA_ERR_INVALID;
-    if (!conf._inCsrFile.empty()) {
-        int64_t *dPtr = nullptr;
-        int32_t *dSrc = nullptr;
-        rc = hipMalloc(&dPtr, (g.n + 1) * sizeof(int64_t)) == hipSuccess &&
-                     hipMalloc(&dSrc, g.n * g.d * sizeof(int32_t)) == hipSuccess
-                 ? PPA_OK
-                 : PPA_ERR_NOMEM;
-        if (!rc) rc = ppa_in_csr(_ctx, &g, _dGraph, dPtr, dSrc, nullptr);
-        int64_t nnz = 0;
-        if (!rc && hipMemcpy(&nnz, dPtr + g.n, sizeof(nnz), hipMemcpyDeviceToHost) != hipSuccess)
-            rc = PPA_ERR_DEVICE;
-        if (!rc && !writeCsr(conf._inCsrFile, csrHeader(g, (uint64_t)nnz, 1), dPtr, dSrc, true))
-            rc = PPA_ERR_INVALID;
-        if (dPtr) (void)hipFree(dPtr);
-        if (dSrc) (void)hipFree(dSrc);
-        if (rc) return rc;
-    }
+    if ((rc = writeDeviceGraphFiles(conf, _ctx, g, _dGraph, _dRowPtr)) != PPA_OK) return rc;
     std::cout << "Ended" << std::endl;
     return PPA_OK;
 }
@@ -250,30 +297,63 @@ int cuPPA::printDegreeDistribution(const ExperimentConfiguration &conf)
     if (!_dGraph || !_ctx) return PPA_ERR_INVALID;
     std::printf("Generating Degree Distribution\n");
     const ppa_graph_desc g = desc(conf);
-    uint32_t *dDeg = nullptr;
-    uint64_t *dMax = nullptr, *dHist = nullptr;
-    int rc = PPA_OK;
-    if (hipMalloc(&dDeg, g.n * sizeof(uint32_t)) != hipSuccess || hipMalloc(&dMax, sizeof(uint64_t)) != hipSuccess)
-        rc = PPA_ERR_NOMEM;
-    uint64_t mx = 0;
-    if (!rc) rc = ppa_degree_histogram(_ctx, &g, _dGraph, dDeg, nullptr, 0, dMax, nullptr);
-    if (!rc && hipMemcpy(&mx, dMax, sizeof(mx), hipMemcpyDeviceToHost) != hipSuccess) rc = PPA_ERR_DEVICE;
-    std::vector<uint64_t> hist(mx + 1, 0);
-    if (!rc && hipMalloc(&dHist, hist.size() * sizeof(uint64_t)) != hipSuccess) rc = PPA_ERR_NOMEM;
-    if (!rc) rc = ppa_degree_histogram(_ctx, &g, _dGraph, dDeg, dHist, hist.size(), dMax, nullptr);
-    if (!rc && hipMemcpy(hist.data(), dHist, hist.size() * sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess)
-        rc = PPA_ERR_DEVICE;
-    if (!rc) {
-        // the reference prints the degrees of its seed nodes (cuPPA.cu:136-139)
-        std::vector<uint32_t> seedDeg(_M);
-        if (hipMemcpy(seedDeg.data(), dDeg, _M * sizeof(uint32_t), hipMemcpyDeviceToHost) == hipSuccess)
-            for (uint32_t u = 0; u < _M; ++u) std::cout << u << "\t" << seedDeg[u] << std::endl;
-        if (!writeDegreeCsv(joinPath(conf._outputDir, conf._distfile1), hist)) rc = PPA_ERR_INVALID;
+    std::vector<uint64_t> hist;
+    std::vector<uint32_t> seedDeg;
+    int rc = deviceDegreeHistogram(_ctx, g, _dGraph, &hist, &seedDeg);
+    if (rc) return rc;
+    // the reference prints the degrees of its seed nodes (cuPPA.cu:136-139)
+    for (uint32_t u = 0; u < _M; ++u) std::cout << u << "\t" << seedDeg[u] << std::endl;
+    return writeDegreeCsv(joinPath(conf._outputDir, conf._distfile1), hist) ? PPA_OK : PPA_ERR_INVALID;
+}
+
+// ---- SerialBA -----------------------------------------------------------------
+
+SerialBA::~SerialBA()
+{
+    if (_dGraph) (void)hipFree(_dGraph);
+    if (_dRowPtr) (void)hipFree(_dRowPtr);
+    if (_ctx) ppa_ctx_destroy(_ctx);
+}
+
+int SerialBA::generateGraph(const ExperimentConfiguration &conf)
+{
+    const std::string METHOD = "SBA";
+    std::cout << METHOD << std::endl;
+    std::ofstream log(conf._outputDir + METHOD + ".log");  // SerialBA.cpp:72-81
+    log << METHOD << std::endl << _N << std::endl << _M << std::endl;
+    ppa_graph_desc g{};
+    g.n = (uint64_t)_N;
+    g.d = _M;
+    g.model = PPA_MODEL_BA;
+    g.seed = conf._seed;
+    int rc = ppa_check_desc(&g);
+    if (rc) return rc;
+    // SerialBA.cpp:24-37: the graph's size, then the allocation
+    log << "Required Memory for Graph: " << (g.n * g.d * 4.0 + (g.n + 1) * 8.0) / (1024.0 * 1024.0) << std::endl;
+    if (!_ctx && (rc = ppa_ctx_create(&_ctx, conf._device)) != PPA_OK) return rc;
+    if (!_dGraph) {
+        if (hipMalloc(&_dGraph, g.n * g.d * sizeof(int32_t)) != hipSuccess) {
+            log << "Not Enough Memory!" << std::endl;
+            return PPA_ERR_NOMEM;
+        }
+        if (hipMalloc(&_dRowPtr, (g.n + 1) * sizeof(int64_t)) != hipSuccess) return PPA_ERR_NOMEM;
     }
-    if (dDeg) (void)hipFree(dDeg);
-    if (dMax) (void)hipFree(dMax);
-    if (dHist) (void)hipFree(dHist);
-    return rc;
+    log << "Memory Allocated" << std::endl;
+    if ((rc = ppa_generate(_ctx, &g, _dGraph, _dRowPtr, nullptr)) != PPA_OK) return rc;
+    ppa_stats st{};
+    ppa_get_stats(_ctx, &st);
+    // SerialBA.cpp:157-162
+    std::cout << "Computation Time : " << st.total_ms * 1e-3 << std::endl;
+    log << "Computation Time : " << st.total_ms * 1e-3 << std::endl;
+    if (conf._FLAG_SHOW_DEGREE_DISTRIBUTION) {
+        // PAHelper.hpp:79-128
+        std::cout << "MAXIMUM SIZE: " << g.n * g.d * 2 << std::endl;
+        std::vector<uint64_t> hist;
+        if ((rc = deviceDegreeHistogram(_ctx, g, _dGraph, &hist, nullptr)) != PPA_OK) return rc;
+        std::printf("Degree of each vertex is computed!\n");
+        if (!writeDegreeCsv(joinPath(conf._outputDir, conf._distfile1), hist)) return PPA_ERR_INVALID;
+    }
+    return writeDeviceGraphFiles(conf, _ctx, g, _dGraph, _dRowPtr);
 }
 
 // ---- SerialCopyModel ----------------------------------------------------------

@@ -3,8 +3,10 @@
 // cuPPA mirrors the reference class of src/PA/cuPPA/cuPPA.cuh:84-105 (same
 // constructor arguments and entry points) over the C ABI of include/ppa.h;
 // SerialCopyModel mirrors src/PA/SPA/SerialCopyModel.h:26-47 over
-// ppa_cpu_generate. Unlike the reference, a generator keeps its graph until
-// it is destroyed and reports failures as a nonzero return value.
+// ppa_cpu_generate; SerialBA mirrors src/PA/SPA/SerialBA.h:30-51 and runs the
+// Barabasi-Albert model on the GPU (PPA_MODEL_BA). Unlike the reference, a
+// generator keeps its graph until it is destroyed and reports failures as a
+// nonzero return value.
 #pragma once
 
 #include <cstdint>
@@ -59,19 +61,43 @@ private:
     std::vector<int32_t> _G;
 };
 
+class SerialBA {
+public:
+    SerialBA(int64_t n, uint32_t m = 1) : _N(n), _M(m) {}
+    ~SerialBA();
+    SerialBA(const SerialBA &) = delete;
+    SerialBA &operator=(const SerialBA &) = delete;
+    // SerialBA.cpp:68-187 (generateGraphNetworkX) with the degree CSV of
+    // PAHelper::outputSBADegreeDistribution (PAHelper.hpp:69-147).
+    int generateGraph(const ExperimentConfiguration &conf);
+
+    const int32_t *deviceTargets() const { return _dGraph; }
+
+private:
+    int64_t _N;
+    uint32_t _M;
+    int32_t *_dGraph = nullptr;
+    int64_t *_dRowPtr = nullptr;
+    ppa_ctx *_ctx = nullptr;
+};
+
 // Output formats of the reference.
 // "Degree,Count" CSV of cuPPA.cu:148-160 and PAHelper.hpp:54-63.
 bool writeDegreeCsv(const std::string &path, const std::vector<uint64_t> &hist);
 // "u v" edge list of PAHelper::writeGraphToDisk (PAHelper.hpp:163-177): the
-// seed clique's edges once each, then every slot of nodes t >= d+1.
-bool writeEdgeList(const std::string &path, const int32_t *targets, uint64_t n, uint32_t d);
+// seed clique's edges once each, then every slot of nodes t >= d+1. For a
+// Barabasi-Albert graph: every slot of nodes t >= d (the pairs of nodes u >= M
+// that PAHelper.hpp:84-121 counts; the seed self-loops are left out).
+bool writeEdgeList(const std::string &path, const int32_t *targets, uint64_t n, uint32_t d,
+                   uint32_t model = PPA_MODEL_COPY);
 std::string joinPath(const std::string &dir, const std::string &file);
 
 // Binary CSR file (this project's format; the reference writes text only):
 // a 64-byte little-endian header
 //   char magic[8] = "PPACSR1"; uint64 n; uint64 nnz; uint32 d; uint32 kind
 //   (0 = out-edges: row t lists t's targets; 1 = in-edges: row v lists the
-//   sources naming v, ascending); uint64 seed; uint64 p_threshold; uint64 0, 0
+//   sources naming v, ascending); uint64 seed; uint64 p_threshold; uint64 model
+//   (PPA_MODEL_*); uint64 0
 // then int64 offsets[n + 1] and int32 indices[nnz]. Arrays given as device
 // pointers are streamed through pinned staging buffers, the copy of one chunk
 // overlapping the write of the previous one.
@@ -80,7 +106,8 @@ struct CsrHeader {
     uint64_t n, nnz;
     uint32_t d, kind;
     uint64_t seed, p_threshold;
-    uint64_t reserved[2];
+    uint64_t model;
+    uint64_t reserved;
 };
 static_assert(sizeof(CsrHeader) == 64, "CSR header layout");
 bool writeCsr(const std::string &path, const CsrHeader &h, const int64_t *offsets, const int32_t *indices,

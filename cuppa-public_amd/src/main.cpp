@@ -1,6 +1,6 @@
 // main.cpp -- command-line driver (counterpart of src/main.cu:46-260).
 //
-//   ppa_runner -A cuppa|cuppa-adaptive|scm -n N -x M -a p [--seed S] [--show-dd 1] ...
+//   ppa_runner -A cuppa|cuppa-adaptive|scm|sba -n N -x M -a p [--seed S] [--show-dd 1] ...
 //
 // Dispatch follows syntheticNetworkGeneration (main.cu:91-119); an unknown
 // algorithm is reported as "Invalid Algorithm" (ExperimentConfiguration.hpp:
@@ -51,6 +51,10 @@ int main(int argc, char **argv)
     if (conf._algorithm == "scm") {
         SerialCopyModel scm(conf._n1, conf._x1, conf._alpha);
         return report(scm.generateGraph(conf));
+    }
+    if (conf._algorithm == "sba") {
+        SerialBA ba(conf._n1, conf._x1);
+        return report(ba.generateGraph(conf));
     }
     if (conf._algorithm == "cuppa" || conf._algorithm == "cuppa-adaptive") {
         cuPPA cuppa(conf._n1, conf._x1, conf._alpha);

@@ -57,13 +57,30 @@ enum {
 #define PPA_MAX_N 0x7fffffffull  /* targets are int32 (DataTypes.h:11-18) */
 
 /* ---- graph descriptor ---------------------------------------------------- */
-/* One copy-model graph. n >= d + 1, 1 <= d <= PPA_MAX_D, p_threshold <= 2^32.
- * targets_offset / row_ptr_offset are element offsets into the concatenated
- * output buffers; only ppa_generate_batch reads them. */
+/* Models (ppa_graph_desc.model, ppa_request.model):
+ *  PPA_MODEL_COPY  the copy model of `-A cuppa` / `-A scm`: seed clique on
+ *                  nodes 0..d, node t >= d+1 draws a uniform earlier (node,
+ *                  slot) pair and takes the node with probability p, else the
+ *                  slot's target (SerialCopyModel3.cpp:38-78).
+ *  PPA_MODEL_BA    the Barabasi-Albert model of `-A sba` (SerialBA.cpp:68-187):
+ *                  node t's d distinct targets are uniform draws over the 2*d*t
+ *                  endpoints of the edges of nodes 0..t-1, where seed nodes
+ *                  i < d own 2d endpoints each and node d's targets are
+ *                  0..d-1. A uniform endpoint is the source or the target of a
+ *                  uniform (node, slot) pair with probability 1/2 each, so
+ *                  this is the copy model at p = 1/2 over seed rows i < d that
+ *                  hold i (self-loops, as the reference stores them); row d
+ *                  holds 0..d-1 and generation starts at d+1 as in the copy
+ *                  model. p_threshold is ignored (fixed at 2^31). */
+enum { PPA_MODEL_COPY = 0, PPA_MODEL_BA = 1 };
+
+/* One graph. n >= d + 1, 1 <= d <= PPA_MAX_D, p_threshold <= 2^32, model a
+ * PPA_MODEL_*. targets_offset / row_ptr_offset are element offsets into the
+ * concatenated output buffers; only ppa_generate_batch reads them. */
 typedef struct ppa_graph_desc {
     uint64_t n;
     uint32_t d;
-    uint32_t reserved;
+    uint32_t model;         /* PPA_MODEL_* (0 = copy model) */
     uint64_t p_threshold;   /* min(2^32, floor(p * 2^32)); see ppa_p_threshold */
     uint64_t seed;          /* Philox key */
     uint64_t targets_offset;
@@ -135,7 +152,10 @@ int ppa_ctx_sync(ppa_ctx *ctx, void *stream);
 
 /* ---- degree pass: replaces cuPPA::printDegreeDistribution (cuPPA.cu:103-165)
  *      and PAHelper::outputSCMDegreeDistribution (PAHelper.hpp:15-67) ------- */
-/* deg(v) = d + #{slots of nodes t >= d+1 naming v}. d_degree (nullable,
+/* deg(v) = d + #{slots of nodes t >= d+1 naming v} for the copy model; for
+ * PPA_MODEL_BA, deg(v) = d*[v >= d] + #{slots of nodes t >= d naming v}, the
+ * count of PAHelper::outputSBADegreeDistribution (PAHelper.hpp:69-147), which
+ * leaves out the seed self-loops. d_degree (nullable,
  * uint32[n]) receives deg; d_hist (nullable, uint64[hist_len]) receives
  * hist[x] = #{v : deg(v) == x} for x < hist_len; d_max_degree (nullable,
  * one uint64) receives max deg. Waits for the stream. */
@@ -158,7 +178,9 @@ int ppa_in_csr(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_target
  * (ascending), so each seed-clique edge appears once per endpoint and
  * d_ptr[v+1] - d_ptr[v] = deg(v) of ppa_degree_histogram. d_ptr int64[n+1],
  * d_adj int32[2*n*d - (d+1)*d]. Work memory: that of ppa_in_csr plus
- * 8*(n+1) + 4*n*d bytes. PPA_ERR_INVALID if a slot is out of range. */
+ * 8*(n+1) + 4*n*d bytes. PPA_ERR_INVALID if a slot is out of range or the
+ * graph is not a copy-model graph (the lists are defined over the clique
+ * seed). */
 int ppa_undirected_csr(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_targets, int64_t *d_ptr,
                        int32_t *d_adj, void *stream);
 
@@ -219,7 +241,7 @@ typedef struct ppa_request {
     uint64_t id;           /* caller-chosen, unique among live requests */
     uint64_t n;
     uint32_t d;
-    uint32_t reserved;
+    uint32_t model;        /* PPA_MODEL_* */
     uint64_t p_threshold;
     uint64_t seed;
 } ppa_request;
@@ -262,7 +284,7 @@ int ppa_server_replay(ppa_server *srv, const ppa_request *reqs, const double *ar
 
 /* ---- serial CPU path (`-A scm`): replaces SerialCopyModel::generateGraph
  *      (src/PA/SPA/SerialCopyModel3.cpp:11-107) ---------------------------- */
-/* Host memory. Same model and draws as ppa_generate, hence the same graph.
+/* Host memory. Same models and draws as ppa_generate, hence the same graph.
  * retries (nullable) receives the number of rejected attempts. */
 int ppa_cpu_generate(const ppa_graph_desc *desc, int32_t *targets, int64_t *row_ptr, uint64_t *retries);
 /* Host-memory degree pass of PAHelper::outputSCMDegreeDistribution

@@ -20,6 +20,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libppa_oracle.so")
 REF_BIN = os.path.join(HERE, "_ref", "scm_ref")
+REF_BA_BIN = os.path.join(HERE, "_ref", "sba_ref")
 
 MAX_RETRY = 1 << 24
 
@@ -51,6 +52,11 @@ def lib():
     L.oracle_generate.argtypes = [u64, u32, u64, u64, i32p, ctypes.POINTER(ctypes.c_int64),
                                   ctypes.POINTER(u64)]
     L.oracle_generate.restype = ctypes.c_int
+    L.oracle_generate_ba.argtypes = [u64, u32, u64, i32p, ctypes.POINTER(u64)]
+    L.oracle_generate_ba.restype = ctypes.c_int
+    L.oracle_degrees_ba.argtypes = [u64, u32, i32p, ctypes.POINTER(u64), ctypes.POINTER(u64), u64,
+                                    ctypes.POINTER(u64)]
+    L.oracle_degrees_ba.restype = ctypes.c_int
     L.oracle_closure_create.argtypes = [u64, u32, u64, u64]
     L.oracle_closure_create.restype = ctypes.c_void_p
     L.oracle_closure_set_prefix.argtypes = [ctypes.c_void_p, i32p, u64]
@@ -75,6 +81,9 @@ def lib():
     L.oracle_verify_parallel.argtypes = [u64, u32, u64, u64, i32p, u64, u64, ctypes.c_int,
                                          ctypes.POINTER(u64)]
     L.oracle_verify_parallel.restype = ctypes.c_int
+    L.oracle_verify_parallel_model.argtypes = [u32, u64, u32, u64, u64, i32p, u64, u64, ctypes.c_int,
+                                               ctypes.POINTER(u64)]
+    L.oracle_verify_parallel_model.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -114,6 +123,31 @@ def generate(n: int, d: int, T: int, seed: int, with_row_ptr: bool = False):
     if rc != 0:
         raise RuntimeError(f"oracle_generate failed: {rc}")
     return G, rp, int(retries.value)
+
+
+def generate_ba(n: int, d: int, seed: int):
+    """Serial Barabasi-Albert (SerialBA.cpp:68-187) in the reference's
+    endpoint-array form. Returns (targets int32 [n*d], retries)."""
+    G = np.empty(n * d, dtype=np.int32)
+    retries = ctypes.c_uint64()
+    rc = lib().oracle_generate_ba(n, d, seed, _ptr(G, ctypes.c_int32), ctypes.byref(retries))
+    if rc != 0:
+        raise RuntimeError(f"oracle_generate_ba failed: {rc}")
+    return G, int(retries.value)
+
+
+def degrees_ba(n: int, d: int, G: np.ndarray, hist_len: int = 0):
+    """Barabasi-Albert degrees (PAHelper.hpp:69-147): (deg, hist or None, max)."""
+    G = np.ascontiguousarray(G, dtype=np.int32)
+    deg = np.empty(n, dtype=np.uint64)
+    hist = np.empty(max(hist_len, 1), dtype=np.uint64)
+    mx = ctypes.c_uint64()
+    rc = lib().oracle_degrees_ba(n, d, _ptr(G, ctypes.c_int32), _ptr(deg, ctypes.c_uint64),
+                                 _ptr(hist, ctypes.c_uint64) if hist_len else None, hist_len,
+                                 ctypes.byref(mx))
+    if rc != 0:
+        raise RuntimeError(f"oracle_degrees_ba failed: {rc}")
+    return deg, (hist[:hist_len] if hist_len else None), int(mx.value)
 
 
 class Closure:
@@ -166,17 +200,18 @@ NO_BAD_ROW = (1 << 64) - 1
 
 
 def verify_rows(n: int, d: int, T: int, seed: int, G: np.ndarray, first: int = 0, count: int | None = None,
-                threads: int = 1):
+                threads: int = 1, model: int = 0):
     """Row-consistency check of a complete candidate graph G (int32 [n*d]).
     Returns the first row in [first, first+count) that is not what the model
-    produces from G's earlier rows, or None if all pass."""
+    produces from G's earlier rows, or None if all pass. model 1 checks a
+    Barabasi-Albert graph (T ignored)."""
     G = np.ascontiguousarray(G, dtype=np.int32)
     assert G.size == n * d
     if count is None:
         count = n - first
     bad = ctypes.c_uint64()
-    rc = lib().oracle_verify_parallel(n, d, T, seed, _ptr(G, ctypes.c_int32), first, count, threads,
-                                      ctypes.byref(bad))
+    rc = lib().oracle_verify_parallel_model(model, n, d, T, seed, _ptr(G, ctypes.c_int32), first, count, threads,
+                                            ctypes.byref(bad))
     if rc != 0:
         raise RuntimeError(f"oracle_verify_parallel failed: {rc}")
     return None if bad.value == NO_BAD_ROW else int(bad.value)
@@ -251,11 +286,14 @@ def py_p_threshold(p: float) -> int:
     return int(Fraction(p) * (1 << 32))
 
 
-def py_generate(n: int, d: int, T: int, seed: int):
+def py_generate(n: int, d: int, T: int, seed: int, self_loop_seed: bool = False):
+    """Copy model over the clique seed, or, with self_loop_seed, over seed rows
+    i < d that hold i (row d still 0..d-1): the copy-model form of the
+    Barabasi-Albert generator at T = 2^31."""
     G = [[-1] * d for _ in range(n)]
     for i in range(d + 1):
         for j in range(d):
-            G[i][j] = j + (1 if j >= i else 0)
+            G[i][j] = i if self_loop_seed and i < d else j + (1 if j >= i else 0)
     key = (seed & _M32, seed >> 32)
     for t in range(d + 1, n):
         row = G[t]
@@ -287,6 +325,27 @@ def run_reference_scm(n: int, d: int, p: float, workdir: str, timeout: float = 6
             secs = float(line.split(":")[1])
     dd = {}
     with open(os.path.join(workdir, "scm_dd.csv")) as f:
+        next(f)
+        for line in f:
+            a, b = line.strip().split(",")
+            dd[int(a)] = int(b)
+    return dd, secs
+
+
+def run_reference_sba(n: int, d: int, workdir: str, timeout: float = 600.0):
+    """Runs the reference's own serial Barabasi-Albert generator (MT19937-64
+    draws), built by ``make -C oracle ref``. Returns (degree->count dict,
+    compute seconds)."""
+    if not os.path.exists(REF_BA_BIN):
+        raise FileNotFoundError(REF_BA_BIN)
+    out = subprocess.run([REF_BA_BIN, str(n), str(d), workdir + "/"], check=True,
+                         capture_output=True, text=True, timeout=timeout).stdout
+    secs = None
+    for line in out.splitlines():
+        if line.startswith("Computation Time"):
+            secs = float(line.split(":")[1])
+    dd = {}
+    with open(os.path.join(workdir, "sba_dd.csv")) as f:
         next(f)
         for line in f:
             a, b = line.strip().split(",")

@@ -155,6 +155,95 @@ int oracle_generate(uint64_t n, uint32_t d, uint64_t T, uint64_t seed, int32_t *
     return ORACLE_OK;
 }
 
+/* ---- serial Barabasi-Albert (SerialBA.cpp:68-187) ------------------------ */
+/*
+ * The reference keeps every edge as an endpoint pair in E[2*M*N]
+ * (SerialBA.cpp:26-29): seed nodes i < M hold 2M copies of i
+ * (SerialBA.cpp:90-97); node t >= M holds (target_j, t) at E[2Mt + 2j],
+ * E[2Mt + 2j + 1] (SerialBA.cpp:138-145). Node M's targets are 0..M-1 in
+ * order (the BSTVertex seeded at SerialBA.cpp:108-112). Node t+1's targets
+ * are distinct draws of E[UniformMax(Z)] over the Z = 2M(t+1) endpoints
+ * written so far, kept in draw order (SerialBA.cpp:149-155, BSTVertex.h:
+ * 189-235). Here the uniform endpoint index is e = 2R + b from the keyed draw
+ * of attempt r of slot k of node t: R = mulhi64((x1 << 32) | x2, t*M) is a
+ * uniform index below t*M and b = (x0 < 2^31) a fair bit, so e is uniform
+ * below 2Mt. The same retry cap as the copy model applies.
+ *
+ * Output is the targets of every node in the [n][d] layout of the copy model
+ * (E's even entries): rows i < d hold i, row d holds 0..d-1.
+ */
+int oracle_generate_ba(uint64_t n, uint32_t d, uint64_t seed, int32_t *G, uint64_t *total_retries)
+{
+    if (d == 0 || n < (uint64_t)d + 1 || n > 0x7fffffffull || !G) return ORACLE_ERR_ARG;
+    const uint64_t M = d;
+    int32_t *E = (int32_t *)malloc(2 * n * M * sizeof(int32_t));
+    if (!E) return ORACLE_ERR_NOMEM;
+    for (uint64_t i = 0; i < M; ++i)
+        for (uint64_t x = 0; x < 2 * M; ++x) E[2 * M * i + x] = (int32_t)i;
+    for (uint64_t j = 0; j < M; ++j) {
+        E[2 * M * M + 2 * j] = (int32_t)j;
+        E[2 * M * M + 2 * j + 1] = (int32_t)M;
+    }
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint64_t retries = 0;
+    int rc = ORACLE_OK;
+    for (uint64_t t = M + 1; t < n && rc == ORACLE_OK; ++t) {
+        int32_t *pair = E + 2 * M * t;
+        for (uint32_t k = 0; k < M && rc == ORACLE_OK; ++k) {
+            for (uint32_t r = 0;; ++r) {
+                const uint32_t ctr[4] = {(uint32_t)t, (uint32_t)(t >> 32), k, r};
+                uint32_t x[4];
+                oracle_philox4x32_10(ctr, key, x);
+                const uint64_t R = oracle_mulhi64(((uint64_t)x[1] << 32) | x[2], t * M);
+                const uint64_t e = 2 * R + (x[0] < (1u << 31) ? 1 : 0);
+                const int32_t cand = E[e];
+                int dup = 0;
+                for (uint32_t i = 0; i < k; ++i)
+                    if (pair[2 * i] == cand) { dup = 1; break; }
+                if (!dup) {
+                    pair[2 * k] = cand;
+                    pair[2 * k + 1] = (int32_t)t;
+                    break;
+                }
+                if (r + 1 >= ORACLE_MAX_RETRY) { rc = ORACLE_ERR_RETRY; break; }
+                ++retries;
+            }
+        }
+    }
+    if (rc == ORACLE_OK)
+        for (uint64_t s = 0; s < n * M; ++s) G[s] = E[2 * s];
+    free(E);
+    if (total_retries) *total_retries = retries;
+    return rc;
+}
+
+/*
+ * Degrees as PAHelper.hpp:69-147 counts them for the Barabasi-Albert graph:
+ * both endpoints of every pair of nodes u >= M, so
+ * deg[v] = M*[v >= M] + #{slots of nodes u >= M naming v}.
+ */
+int oracle_degrees_ba(uint64_t n, uint32_t d, const int32_t *G, uint64_t *deg_out, uint64_t *hist,
+                      uint64_t hist_len, uint64_t *max_degree)
+{
+    uint64_t *deg = deg_out ? deg_out : (uint64_t *)malloc(n * sizeof(uint64_t));
+    if (!deg) return ORACLE_ERR_NOMEM;
+    memset(deg, 0, n * sizeof(uint64_t));
+    for (uint64_t u = d; u < n; ++u)
+        for (uint32_t j = 0; j < d; ++j) {
+            deg[G[u * d + j]]++;
+            deg[u]++;
+        }
+    uint64_t mx = 0;
+    if (hist) memset(hist, 0, hist_len * sizeof(uint64_t));
+    for (uint64_t v = 0; v < n; ++v) {
+        if (deg[v] > mx) mx = deg[v];
+        if (hist && deg[v] < hist_len) hist[deg[v]]++;
+    }
+    if (max_degree) *max_degree = mx;
+    if (!deg_out) free(deg);
+    return ORACLE_OK;
+}
+
 /* ---- closure oracle: target(t, k) by memoised recursion ------------------ */
 /*
  * Computes single slots of a huge graph without generating the prefix. Every
@@ -388,17 +477,25 @@ int oracle_undirected_csr(uint64_t n, uint32_t d, const int32_t *G, int64_t *ptr
  * differs from the oracle would have all of its copy sources right and so
  * could not pass). Returns the first failing row through *bad (or UINT64_MAX).
  */
-int oracle_verify_rows(uint64_t n, uint32_t d, uint64_t T, uint64_t seed, const int32_t *G,
-                       uint64_t first, uint64_t count, uint64_t *bad)
+/*
+ * model 1 checks a Barabasi-Albert graph in the copy-model form that
+ * oracle_generate_ba is pinned to: seed rows i < d hold i, row d holds
+ * 0..d-1, and the coin threshold is 2^31 (T is ignored).
+ */
+int oracle_verify_rows_model(uint32_t model, uint64_t n, uint32_t d, uint64_t T, uint64_t seed,
+                             const int32_t *G, uint64_t first, uint64_t count, uint64_t *bad)
 {
     *bad = UINT64_MAX;
-    if (d == 0 || d > 4096 || n < (uint64_t)d + 1 || first + count > n) return ORACLE_ERR_ARG;
+    if (d == 0 || d > 4096 || n < (uint64_t)d + 1 || first + count > n || model > 1) return ORACLE_ERR_ARG;
+    if (model == 1) T = 1ull << 31;
     int32_t row[4096];
     for (uint64_t t = first; t < first + count; ++t) {
         const int32_t *g = G + t * d;
         if (t <= d) {
-            for (uint32_t k = 0; k < d; ++k)
-                if (g[k] != (int32_t)(k + (k >= t))) { *bad = t; return ORACLE_OK; }
+            for (uint32_t k = 0; k < d; ++k) {
+                const int32_t want = model == 1 && t < d ? (int32_t)t : (int32_t)(k + (k >= t));
+                if (g[k] != want) { *bad = t; return ORACLE_OK; }
+            }
             continue;
         }
         for (uint32_t k = 0; k < d; ++k) {
@@ -423,9 +520,15 @@ int oracle_verify_rows(uint64_t n, uint32_t d, uint64_t T, uint64_t seed, const 
     return ORACLE_OK;
 }
 
+int oracle_verify_rows(uint64_t n, uint32_t d, uint64_t T, uint64_t seed, const int32_t *G,
+                       uint64_t first, uint64_t count, uint64_t *bad)
+{
+    return oracle_verify_rows_model(0, n, d, T, seed, G, first, count, bad);
+}
+
 typedef struct {
     uint64_t n, T, seed, first, count, bad;
-    uint32_t d;
+    uint32_t d, model;
     const int32_t *G;
     int rc;
 } verify_job;
@@ -433,13 +536,22 @@ typedef struct {
 static void *verify_worker(void *arg)
 {
     verify_job *j = (verify_job *)arg;
-    j->rc = oracle_verify_rows(j->n, j->d, j->T, j->seed, j->G, j->first, j->count, &j->bad);
+    j->rc = oracle_verify_rows_model(j->model, j->n, j->d, j->T, j->seed, j->G, j->first, j->count, &j->bad);
     return NULL;
 }
+
+int oracle_verify_parallel_model(uint32_t model, uint64_t n, uint32_t d, uint64_t T, uint64_t seed,
+                                 const int32_t *G, uint64_t first, uint64_t count, int threads, uint64_t *bad);
 
 /* oracle_verify_rows over [first, first+count) split across `threads` threads. */
 int oracle_verify_parallel(uint64_t n, uint32_t d, uint64_t T, uint64_t seed, const int32_t *G,
                            uint64_t first, uint64_t count, int threads, uint64_t *bad)
+{
+    return oracle_verify_parallel_model(0, n, d, T, seed, G, first, count, threads, bad);
+}
+
+int oracle_verify_parallel_model(uint32_t model, uint64_t n, uint32_t d, uint64_t T, uint64_t seed,
+                                 const int32_t *G, uint64_t first, uint64_t count, int threads, uint64_t *bad)
 {
     *bad = UINT64_MAX;
     if (threads < 1) threads = 1;
@@ -452,7 +564,7 @@ int oracle_verify_parallel(uint64_t n, uint32_t d, uint64_t T, uint64_t seed, co
         uint64_t lo = first + (uint64_t)i * per;
         if (lo >= first + count) break;
         uint64_t hi = lo + per < first + count ? lo + per : first + count;
-        jobs[i] = (verify_job){n, T, seed, lo, hi - lo, UINT64_MAX, d, G, 0};
+        jobs[i] = (verify_job){n, T, seed, lo, hi - lo, UINT64_MAX, d, model, G, 0};
         if (pthread_create(&tid[i], NULL, verify_worker, &jobs[i]) != 0) {
             verify_worker(&jobs[i]);
             tid[i] = 0;

@@ -8,6 +8,9 @@
   reference's own serial copy model (src/PA/SPA/SerialCopyModel3.cpp built by
   `make -C oracle ref`; MT19937-64 draws with its default seed), used for
   statistical parity of our degree distributions.
+* ref_dd_ba_n<N>_d<D>.csv -- the same from the reference's serial
+  Barabasi-Albert generator (src/PA/SPA/SerialBA.cpp, also built by
+  `make -C oracle ref`).
 """
 import hashlib
 import json
@@ -26,6 +29,7 @@ import oracle  # noqa: E402
 TINY = [(2, 1, 0.5, 0), (5, 2, 0.5, 1), (12, 3, 0.3, 2), (20, 4, 0.0, 3), (24, 4, 1.0, 4),
         (32, 5, 0.5, 5), (32, 8, 0.2, 6), (30, 16, 0.6, 7), (32, 1, 0.5, 2**40 + 1)]
 REF_DD = [(1_000_000, 4, 0.5), (1_000_000, 8, 0.2), (300_000, 16, 0.5), (300_000, 2, 0.9)]
+REF_DD_BA = [(1_000_000, 1), (1_000_000, 4), (300_000, 16)]
 
 
 def main():
@@ -51,6 +55,10 @@ def main():
         with tempfile.TemporaryDirectory() as tmp:
             oracle.run_reference_scm(n, d, p, tmp)
             shutil.copy(os.path.join(tmp, "scm_dd.csv"), os.path.join(HERE, f"ref_dd_n{n}_d{d}_p{p}.csv"))
+    for n, d in REF_DD_BA:
+        with tempfile.TemporaryDirectory() as tmp:
+            oracle.run_reference_sba(n, d, tmp)
+            shutil.copy(os.path.join(tmp, "sba_dd.csv"), os.path.join(HERE, f"ref_dd_ba_n{n}_d{d}.csv"))
 
 
 if __name__ == "__main__":

@@ -74,6 +74,13 @@ def test_check_desc(L, n, d, T, ok):
     assert (L.ppa_check_desc(ctypes.byref(desc)) == 0) == ok
 
 
+@pytest.mark.parametrize("model,ok", [(0, True), (1, True), (2, False), (2**31, False)])
+def test_check_desc_model(L, model, ok):
+    import cuppa
+    desc = cuppa.GraphDesc(100, 4, model, 0, 0, 0, 0)
+    assert (L.ppa_check_desc(ctypes.byref(desc)) == 0) == ok
+
+
 def test_null_arguments_rejected(L):
     assert L.ppa_check_desc(None) == -1
     assert L.ppa_ctx_destroy(None) == -1
@@ -108,6 +115,31 @@ def test_cpu_path_matches_oracle(L, oracle_mod, n, d, p, seed):
     assert L.ppa_cpu_degree_histogram(ctypes.byref(desc), G.ctypes.data, deg.ctypes.data, hist.ctypes.data,
                                       hist.size, ctypes.byref(mx)) == 0
     wdeg, whist, wmx = oracle_mod.degrees(n, d, want, hist_len=4096)
+    assert np.array_equal(deg.astype(np.uint64), wdeg)
+    assert np.array_equal(hist, whist)
+    assert mx.value == wmx
+
+
+@pytest.mark.parametrize("n,d,seed", [(2, 1, 0), (5, 4, 1), (20_000, 1, 2), (50_000, 8, 11),
+                                      (30_000, 16, 3), (1_000_000, 4, 42)])
+def test_cpu_path_ba_matches_oracle(L, oracle_mod, n, d, seed):
+    """PPA_MODEL_BA through the product's serial loop (copy-model form over
+    self-loop seed rows) against the oracle's endpoint-array restatement of
+    SerialBA.cpp; p_threshold is ignored."""
+    import cuppa
+    desc = cuppa.make_desc(n, d, p_thresh=12345, seed=seed, model=cuppa.MODEL_BA)
+    G = np.empty(n * d, dtype=np.int32)
+    retries = ctypes.c_uint64()
+    assert L.ppa_cpu_generate(ctypes.byref(desc), G.ctypes.data, None, ctypes.byref(retries)) == 0
+    want, want_retries = oracle_mod.generate_ba(n, d, seed)
+    assert np.array_equal(G, want)
+    assert retries.value == want_retries
+    deg = np.empty(n, dtype=np.uint32)
+    hist = np.empty(8192, dtype=np.uint64)
+    mx = ctypes.c_uint64()
+    assert L.ppa_cpu_degree_histogram(ctypes.byref(desc), G.ctypes.data, deg.ctypes.data, hist.ctypes.data,
+                                      hist.size, ctypes.byref(mx)) == 0
+    wdeg, whist, wmx = oracle_mod.degrees_ba(n, d, want, hist_len=8192)
     assert np.array_equal(deg.astype(np.uint64), wdeg)
     assert np.array_equal(hist, whist)
     assert mx.value == wmx
