@@ -314,6 +314,23 @@ int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t
     return PPA_OK;
 }
 
+int ppa_histogram_of_degrees(ppa_ctx *ctx, uint64_t n, const uint32_t *d_degree, uint64_t *d_hist,
+                             uint64_t hist_len, uint64_t *d_max_degree, void *stream_)
+{
+    if (!ctx || !d_degree || n == 0 || n > PPA_MAX_N) return PPA_ERR_INVALID;
+    if (d_hist == nullptr && hist_len != 0) return PPA_ERR_INVALID;
+    DeviceGuard g(ctx->device);
+    if (!g.ok()) return PPA_ERR_DEVICE;
+    hipStream_t stream = (hipStream_t)stream_;
+    PPA_HIP(launch_reset_scratch(ctx->scratch, 0, stream));
+    PPA_HIP(launch_degree_histogram(d_degree, n, d_hist, hist_len, ctx->scratch, stream));
+    if (d_max_degree)
+        PPA_HIP(hipMemcpyAsync(d_max_degree, &ctx->scratch->max_degree, sizeof(uint64_t),
+                               hipMemcpyDeviceToDevice, stream));
+    PPA_HIP(hipStreamSynchronize(stream));
+    return PPA_OK;
+}
+
 int ppa_in_csr(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_targets, int64_t *d_in_ptr,
                int32_t *d_in_src, void *stream_)
 {

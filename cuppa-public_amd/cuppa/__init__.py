@@ -107,6 +107,7 @@ _SIGNATURES = [
     ("ppa_ctx_sync", ctypes.c_int, [_VP, _VP]),
     ("ppa_degree_histogram", ctypes.c_int,
      [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP, ctypes.c_uint64, _VP, _VP]),
+    ("ppa_histogram_of_degrees", ctypes.c_int, [_VP, ctypes.c_uint64, _VP, _VP, ctypes.c_uint64, _VP, _VP]),
     ("ppa_in_csr", ctypes.c_int, [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP, _VP]),
     ("ppa_undirected_csr", ctypes.c_int, [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP, _VP]),
     ("ppa_batch_create", ctypes.c_int,
@@ -283,25 +284,32 @@ class Context:
         return ptr, adj
 
     def degree_histogram(self, g: Graph, hist_len: int | None = None, want_degree: bool = False, stream=None):
-        """Returns (hist uint64 tensor, max_degree int, degree tensor or None)."""
+        """Returns (hist uint64 tensor, max_degree int, degree tensor or None).
+        Without hist_len the slots are counted once (ppa_degree_histogram) and
+        the histogram is sized from the maximum (ppa_histogram_of_degrees)."""
         import torch
         dev = torch.device("cuda", self.device)
         desc = make_desc(g.n, g.d, p_thresh=0, model=g.model)
-        deg = torch.empty(g.n, dtype=torch.int32, device=dev) if want_degree else None
+        deg = torch.empty(g.n, dtype=torch.int32, device=dev) if (want_degree or hist_len is None) else None
         mx = torch.zeros(1, dtype=torch.int64, device=dev)
+        sh = _stream_handle(stream)
         if hist_len is None:
-            # first pass only for the maximum degree
+            _check(lib().ppa_degree_histogram(self._h, ctypes.byref(desc), ctypes.c_void_p(g.targets.data_ptr()),
+                                              ctypes.c_void_p(deg.data_ptr()), None, 0,
+                                              ctypes.c_void_p(mx.data_ptr()), sh), "ppa_degree_histogram")
+            hist_len = int(mx.item()) + 1
+            hist = torch.zeros(hist_len, dtype=torch.int64, device=dev)
+            _check(lib().ppa_histogram_of_degrees(self._h, g.n, ctypes.c_void_p(deg.data_ptr()),
+                                                  ctypes.c_void_p(hist.data_ptr()), hist_len,
+                                                  ctypes.c_void_p(mx.data_ptr()), sh), "ppa_histogram_of_degrees")
+        else:
+            hist = torch.zeros(hist_len, dtype=torch.int64, device=dev)
             _check(lib().ppa_degree_histogram(self._h, ctypes.byref(desc), ctypes.c_void_p(g.targets.data_ptr()),
                                               ctypes.c_void_p(deg.data_ptr()) if deg is not None else None,
-                                              None, 0, ctypes.c_void_p(mx.data_ptr()), _stream_handle(stream)),
-                   "ppa_degree_histogram")
-            hist_len = int(mx.item()) + 1
-        hist = torch.zeros(hist_len, dtype=torch.int64, device=dev)
-        _check(lib().ppa_degree_histogram(self._h, ctypes.byref(desc), ctypes.c_void_p(g.targets.data_ptr()),
-                                          ctypes.c_void_p(deg.data_ptr()) if deg is not None else None,
-                                          ctypes.c_void_p(hist.data_ptr()), hist_len,
-                                          ctypes.c_void_p(mx.data_ptr()), _stream_handle(stream)),
-               "ppa_degree_histogram")
+                                              ctypes.c_void_p(hist.data_ptr()), hist_len,
+                                              ctypes.c_void_p(mx.data_ptr()), sh), "ppa_degree_histogram")
+        if not want_degree:
+            deg = None
         return hist, int(mx.item()), deg
 
 

@@ -117,7 +117,8 @@ CsrHeader csrHeader(const ppa_graph_desc &g, uint64_t nnz, uint32_t kind)
 namespace {
 
 // Degree histogram of a generated graph on the device: ppa_degree_histogram
-// once for the maximum degree, then into a histogram of that length.
+// for the degrees and their maximum, then ppa_histogram_of_degrees into a
+// histogram of that length.
 // seedDeg (nullable) receives the degrees of nodes 0..d-1.
 int deviceDegreeHistogram(ppa_ctx *ctx, const ppa_graph_desc &g, const int32_t *dGraph,
                           std::vector<uint64_t> *hist, std::vector<uint32_t> *seedDeg)
@@ -132,7 +133,7 @@ int deviceDegreeHistogram(ppa_ctx *ctx, const ppa_graph_desc &g, const int32_t *
     if (!rc && hipMemcpy(&mx, dMax, sizeof(mx), hipMemcpyDeviceToHost) != hipSuccess) rc = PPA_ERR_DEVICE;
     if (!rc) hist->assign(mx + 1, 0);
     if (!rc && hipMalloc(&dHist, hist->size() * sizeof(uint64_t)) != hipSuccess) rc = PPA_ERR_NOMEM;
-    if (!rc) rc = ppa_degree_histogram(ctx, &g, dGraph, dDeg, dHist, hist->size(), dMax, nullptr);
+    if (!rc) rc = ppa_histogram_of_degrees(ctx, g.n, dDeg, dHist, hist->size(), dMax, nullptr);
     if (!rc && hipMemcpy(hist->data(), dHist, hist->size() * sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess)
         rc = PPA_ERR_DEVICE;
     if (!rc && seedDeg) {

@@ -162,6 +162,13 @@ int ppa_ctx_sync(ppa_ctx *ctx, void *stream);
 int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_targets,
                          uint32_t *d_degree, uint64_t *d_hist, uint64_t hist_len,
                          uint64_t *d_max_degree, void *stream);
+/* The histogram step alone (cuPPA.cu:147-158), over degrees a previous
+ * ppa_degree_histogram call left in d_degree (uint32[n]): d_hist[x] =
+ * #{v : deg(v) == x} for x < hist_len (nullable with hist_len 0), and the
+ * maximum degree. Lets a caller size the histogram from the maximum without
+ * counting the slots twice. Waits for the stream. */
+int ppa_histogram_of_degrees(ppa_ctx *ctx, uint64_t n, const uint32_t *d_degree, uint64_t *d_hist,
+                             uint64_t hist_len, uint64_t *d_max_degree, void *stream);
 
 /* ---- in-edge CSR (no counterpart in the reference, which never builds one;
  *      the consumer of `targets` that the reference's users build next) ---- */

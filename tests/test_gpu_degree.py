@@ -48,3 +48,33 @@ def test_degrees_match_counting(ctx, n, d, power):
     assert np.array_equal(got, want)
     assert mx == int(want.max())
     assert np.array_equal(hist.cpu().numpy(), np.bincount(want))
+
+
+def test_histogram_of_degrees(ctx):
+    """ppa_histogram_of_degrees over degrees left by ppa_degree_histogram:
+    same histogram and maximum, truncated below hist_len, maximum alone with
+    no histogram."""
+    import ctypes
+    import torch
+    import cuppa
+    n, d = 3 << 20, 5
+    G = _targets(n, d, 11, 3)
+    g = cuppa.Graph(n, d, torch.from_numpy(G).cuda(), None)
+    hist, mx, deg = ctx.degree_histogram(g, want_degree=True)
+    want = np.bincount(deg.cpu().numpy().astype(np.int64))
+    assert np.array_equal(hist.cpu().numpy(), want) and mx == want.size - 1
+    L = cuppa.lib()
+    out_max = torch.zeros(1, dtype=torch.int64, device="cuda")
+    for hist_len in (0, 7, want.size, want.size + 100):
+        h = torch.full((max(hist_len, 1),), -1, dtype=torch.int64, device="cuda")
+        rc = L.ppa_histogram_of_degrees(ctx.handle, n, ctypes.c_void_p(deg.data_ptr()),
+                                        ctypes.c_void_p(h.data_ptr()) if hist_len else None, hist_len,
+                                        ctypes.c_void_p(out_max.data_ptr()), None)
+        assert rc == 0
+        assert int(out_max.item()) == mx
+        if hist_len:
+            w = np.zeros(hist_len, np.int64)
+            k = min(hist_len, want.size)
+            w[:k] = want[:k]
+            assert np.array_equal(h.cpu().numpy(), w)
+    assert L.ppa_histogram_of_degrees(ctx.handle, n, None, None, 0, None, None) == -1
