@@ -143,6 +143,68 @@ def copy_bandwidth_gbs(torch, nbytes: int = 1 << 32) -> float:
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
+def gather_ceiling(torch, table_bytes: int = 32 << 30, reads: int = 1 << 28) -> dict:
+    """Random 4-byte reads per second on this GPU over an int32 table of
+    `table_bytes` (each read is one 64-byte HBM request, like the generator's
+    copy reads): the hand-written probe kernel of lib/libppa_probe.so (plain
+    and non-temporal loads, best of 3 each) and, for comparison, torch's
+    index_select at `reads` uniform random positions."""
+    import ctypes
+    src = torch.zeros(table_bytes // 4, dtype=torch.int32, device="cuda")
+    out = {"table_bytes": table_bytes}
+    probe_path = os.path.join(ROOT, "cuppa-public_amd", "lib", "libppa_probe.so")
+    if os.path.exists(probe_path):
+        P = ctypes.CDLL(probe_path)
+        P.ppa_probe_gather.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.POINTER(ctypes.c_float)]
+        P.ppa_probe_gather.restype = ctypes.c_int
+        blocks, threads, iters = 2048, 256, 512
+        sink = torch.empty(blocks * threads, dtype=torch.int32, device="cuda")
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for nt in (0, 1):
+            best = float("inf")
+            for _ in range(4):  # the first launch warms up
+                ms = ctypes.c_float()
+                rc = P.ppa_probe_gather(ctypes.c_void_p(src.data_ptr()), src.numel(), blocks, threads, iters, nt,
+                                        ctypes.c_void_p(sink.data_ptr()), stream, ctypes.byref(ms))
+                if rc != 0:
+                    raise RuntimeError(f"ppa_probe_gather failed: {rc}")
+                best = min(best, ms.value)
+            out["probe_nt" if nt else "probe_plain"] = blocks * threads * iters / (best * 1e-3)
+        del sink
+    idx = torch.randint(0, src.numel(), (reads,), dtype=torch.int64, device="cuda")
+    got = src[idx]
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(3):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        torch.index_select(src, 0, idx, out=got)
+        e.record()
+        torch.cuda.synchronize()
+        best = min(best, s.elapsed_time(e))
+    out["torch_index_select"] = reads / (best * 1e-3)
+    del src, idx, got
+    torch.cuda.empty_cache()
+    out["requests_per_s"] = max(v for k, v in out.items() if k != "table_bytes")
+    return out
+
+
+def read_requests(profile_dir: str, workload: str):
+    """TCC read requests per launch of the generation kernel, from the
+    committed rocprofv3 --pmc summary for this workload (None if absent)."""
+    path = os.path.join(profile_dir, f"generate_kernel_{workload}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            m = re.match(r"\s*tcc_ea_read_requests\s*[:=]\s*([0-9.eE+]+)", line)
+            if m:
+                return float(m.group(1))
+    return None
+
+
 def batch_specs(count: int = 4096):
     """The ensemble: n log-uniform in [1e3, 1e7], d uniform in 1..16, p uniform
     in [0.05, 0.95], Philox seed 1000 + i; the meta-draw is numpy's Philox
@@ -298,6 +360,8 @@ def main():
                     help="nodes of the CPU baseline sample (0 = skip)")
     ap.add_argument("--degree", action="store_true", help="also time the degree-histogram pass")
     ap.add_argument("--in-csr", action="store_true", help="also time the in-edge CSR build")
+    ap.add_argument("--no-gather-ceiling", action="store_true",
+                    help="skip the random-gather ceiling measurement (needs 35 GB of free HBM)")
     ap.add_argument("--ba", action="store_true",
                     help="also time the Barabasi-Albert model (-A sba) at the workload's n and d, "
                          "and the reference's serial BA on a prefix")
@@ -440,6 +504,14 @@ def main():
 
     del targets, row_ptr
     torch.cuda.empty_cache()
+    requests = read_requests(os.path.join(ROOT, "profiles"), args.workload)
+    request_roof = None
+    if requests and not args.no_gather_ceiling:
+        ceil = gather_ceiling(torch)
+        rate = requests / (k_ms * 1e-3)
+        request_roof = {"read_requests_per_launch": requests, "achieved_requests_per_s": rate,
+                        "gather_ceiling": ceil, "frac": rate / ceil["requests_per_s"],
+                        "unit": "64-byte read requests/s"}
     cpu = cpu_baseline(n, d, p, args.cpu_prefix) if args.cpu_prefix else None
 
     line = {
@@ -463,6 +535,9 @@ def main():
                      "algorithmic_bytes_per_edge": bpe["generate_kernel"],
                      "kernel_ms": k_ms},
         "cpu_baseline": cpu,
+        # the kernel's 64-byte read requests (committed PMC profile) per second
+        # against a plain random gather's on the same GPU
+        "request_roofline": request_roof,
         "call_roofline_frac": bpe["call"] * edges / (c_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "call_ms": c_ms,
         "measured_copy_gbs": copy_gbs,

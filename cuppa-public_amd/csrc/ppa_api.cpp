@@ -218,6 +218,7 @@ int ppa_generate_async(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targ
     int rc = ppa_check_desc(desc);
     if (rc) return rc;
     if (((uintptr_t)d_targets & 15u) != 0) return PPA_ERR_INVALID;
+    if (ctx->pending) return PPA_ERR_AGAIN;  // the previous launch still owns the scratch
     DeviceGuard g(ctx->device);
     if (!g.ok()) return PPA_ERR_DEVICE;
     hipStream_t stream = (hipStream_t)stream_;
@@ -281,6 +282,7 @@ int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t
     int rc = ppa_check_desc(desc);
     if (rc) return rc;
     if (d_hist == nullptr && hist_len != 0) return PPA_ERR_INVALID;
+    if (ctx->pending) return PPA_ERR_AGAIN;
     DeviceGuard g(ctx->device);
     if (!g.ok()) return PPA_ERR_DEVICE;
     hipStream_t stream = (hipStream_t)stream_;
@@ -319,6 +321,7 @@ int ppa_histogram_of_degrees(ppa_ctx *ctx, uint64_t n, const uint32_t *d_degree,
 {
     if (!ctx || !d_degree || n == 0 || n > PPA_MAX_N) return PPA_ERR_INVALID;
     if (d_hist == nullptr && hist_len != 0) return PPA_ERR_INVALID;
+    if (ctx->pending) return PPA_ERR_AGAIN;
     DeviceGuard g(ctx->device);
     if (!g.ok()) return PPA_ERR_DEVICE;
     hipStream_t stream = (hipStream_t)stream_;
@@ -337,6 +340,7 @@ int ppa_in_csr(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_target
     if (!ctx || !d_targets || !d_in_ptr || !d_in_src) return PPA_ERR_INVALID;
     int rc = ppa_check_desc(desc);
     if (rc) return rc;
+    if (ctx->pending) return PPA_ERR_AGAIN;
     DeviceGuard g(ctx->device);
     if (!g.ok()) return PPA_ERR_DEVICE;
     hipStream_t stream = (hipStream_t)stream_;
@@ -354,6 +358,7 @@ int ppa_undirected_csr(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *
     if (!ctx || !d_targets || !d_ptr || !d_adj) return PPA_ERR_INVALID;
     int rc = ppa_check_desc(desc);
     if (rc) return rc;
+    if (ctx->pending) return PPA_ERR_AGAIN;
     DeviceGuard g(ctx->device);
     if (!g.ok()) return PPA_ERR_DEVICE;
     if (desc->model != PPA_MODEL_COPY) return PPA_ERR_INVALID;

@@ -145,7 +145,10 @@ int ppa_get_stats(ppa_ctx *ctx, ppa_stats *out);
  * returns the kernel's status. */
 int ppa_generate(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets,
                  int64_t *d_row_ptr, void *stream);
-/* Same work without the final wait; the status is read by ppa_ctx_sync. */
+/* Same work without the final wait; the status is read by ppa_ctx_sync on the
+ * same stream. Until then the launch owns the context's scratch: another
+ * ppa_generate*, ppa_degree_histogram, ppa_histogram_of_degrees, ppa_in_csr
+ * or ppa_undirected_csr call on the context returns PPA_ERR_AGAIN. */
 int ppa_generate_async(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets,
                        int64_t *d_row_ptr, void *stream);
 int ppa_ctx_sync(ppa_ctx *ctx, void *stream);

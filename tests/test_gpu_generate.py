@@ -79,3 +79,27 @@ def test_degree_histogram(ctx, oracle_mod):
     assert mx == mx_want
     assert np.array_equal(deg.cpu().numpy().astype(np.uint64), deg_want)
     assert np.array_equal(hist.cpu().numpy(), hist_want)
+
+
+def test_async_launch_owns_the_context(ctx, oracle_mod):
+    """While an async launch is pending, calls that share the context's
+    scratch return PPA_ERR_AGAIN instead of racing it; ppa_ctx_sync ends it."""
+    import ctypes
+    import torch
+    import cuppa
+    n, d, p, seed = 300_000, 6, 0.4, 5
+    T = oracle_mod.p_threshold(p)
+    desc = cuppa.make_desc(n, d, p_thresh=T, seed=seed)
+    t1 = torch.empty(n * d, dtype=torch.int32, device="cuda")
+    t2 = torch.empty(n * d, dtype=torch.int32, device="cuda")
+    L = cuppa.lib()
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert L.ppa_generate_async(ctx.handle, ctypes.byref(desc), ctypes.c_void_p(t1.data_ptr()), None, s) == 0
+    assert L.ppa_generate_async(ctx.handle, ctypes.byref(desc), ctypes.c_void_p(t2.data_ptr()), None, s) == -9
+    assert L.ppa_degree_histogram(ctx.handle, ctypes.byref(desc), ctypes.c_void_p(t1.data_ptr()), None, None, 0,
+                                  None, s) == -9
+    assert L.ppa_ctx_sync(ctx.handle, s) == 0
+    want, _, _ = oracle_mod.generate(n, d, T, seed)
+    assert np.array_equal(t1.cpu().numpy(), want)
+    assert L.ppa_generate(ctx.handle, ctypes.byref(desc), ctypes.c_void_p(t2.data_ptr()), None, s) == 0
+    assert np.array_equal(t2.cpu().numpy(), want)
