@@ -146,9 +146,12 @@ def copy_bandwidth_gbs(torch, nbytes: int = 1 << 32) -> float:
 def gather_ceiling(torch, table_bytes: int = 32 << 30, reads: int = 1 << 28) -> dict:
     """Random 4-byte reads per second on this GPU over an int32 table of
     `table_bytes` (each read is one 64-byte HBM request, like the generator's
-    copy reads): the hand-written probe kernel of lib/libppa_probe.so (plain
-    and non-temporal loads, best of 3 each) and, for comparison, torch's
-    index_select at `reads` uniform random positions."""
+    copy reads): the hand-written probe kernels of lib/libppa_probe.so (plain
+    and non-temporal loads into registers or into LDS, best of 3 each; and the
+    LDS form with one 2 KB block of row stores per wavefront and batch of
+    eight reads, the generator's own mix of reads and publication) and, for
+    comparison, torch's index_select at `reads` uniform random positions.
+    requests_per_s is the best pure-read rate."""
     import ctypes
     src = torch.zeros(table_bytes // 4, dtype=torch.int32, device="cuda")
     out = {"table_bytes": table_bytes}
@@ -162,16 +165,17 @@ def gather_ceiling(torch, table_bytes: int = 32 << 30, reads: int = 1 << 28) -> 
         blocks, threads, iters = 2048, 256, 512
         sink = torch.empty(blocks * threads, dtype=torch.int32, device="cuda")
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-        for nt in (0, 1):
+        for mode, name in enumerate(("probe_plain", "probe_nt", "probe_lds", "probe_lds_nt",
+                                     "probe_lds_nt_with_row_stores")):
             best = float("inf")
             for _ in range(4):  # the first launch warms up
                 ms = ctypes.c_float()
-                rc = P.ppa_probe_gather(ctypes.c_void_p(src.data_ptr()), src.numel(), blocks, threads, iters, nt,
+                rc = P.ppa_probe_gather(ctypes.c_void_p(src.data_ptr()), src.numel(), blocks, threads, iters, mode,
                                         ctypes.c_void_p(sink.data_ptr()), stream, ctypes.byref(ms))
                 if rc != 0:
                     raise RuntimeError(f"ppa_probe_gather failed: {rc}")
                 best = min(best, ms.value)
-            out["probe_nt" if nt else "probe_plain"] = blocks * threads * iters / (best * 1e-3)
+            out[name] = blocks * threads * iters / (best * 1e-3)
         del sink
     idx = torch.randint(0, src.numel(), (reads,), dtype=torch.int64, device="cuda")
     got = src[idx]
@@ -187,7 +191,7 @@ def gather_ceiling(torch, table_bytes: int = 32 << 30, reads: int = 1 << 28) -> 
     out["torch_index_select"] = reads / (best * 1e-3)
     del src, idx, got
     torch.cuda.empty_cache()
-    out["requests_per_s"] = max(v for k, v in out.items() if k != "table_bytes")
+    out["requests_per_s"] = max(v for k, v in out.items() if k not in ("table_bytes", "probe_lds_nt_with_row_stores"))
     return out
 
 
@@ -512,6 +516,9 @@ def main():
         request_roof = {"read_requests_per_launch": requests, "achieved_requests_per_s": rate,
                         "gather_ceiling": ceil, "frac": rate / ceil["requests_per_s"],
                         "unit": "64-byte read requests/s"}
+        if "probe_lds_nt_with_row_stores" in ceil:
+            # against reads that share the memory with row stores, as the generator's do
+            request_roof["frac_of_read_write_mix"] = rate / ceil["probe_lds_nt_with_row_stores"]
     cpu = cpu_baseline(n, d, p, args.cpu_prefix) if args.cpu_prefix else None
 
     line = {
@@ -542,7 +549,9 @@ def main():
         "call_ms": c_ms,
         "measured_copy_gbs": copy_gbs,
         "stats": {**{k: last[k] for k in ("draws", "retries", "prefetch_misses", "polls", "stalls", "claims",
-                                          "sleeps", "staged_rows", "grid_blocks", "block_threads")},
+                                          "sleeps", "staged_rows", "passes", "grid_blocks", "block_threads")},
+                  # nodes a wavefront takes on per loop pass (64 lanes)
+                  "nodes_per_pass": last["nodes"] / max(1, last["passes"]),
                   # share of slots whose first copy read found the source not yet published
                   "stall_fraction": last["prefetch_misses"] / max(1, last["slots"])},
         **extra,

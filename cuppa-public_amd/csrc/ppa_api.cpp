@@ -94,6 +94,7 @@ int finish_launch(ppa_ctx *ctx, hipStream_t stream)
     s.claims = host.stats[kStatClaims];
     s.sleeps = host.stats[kStatSleeps];
     s.staged_rows = host.stats[kStatStagedRows];
+    s.passes = host.stats[kStatPasses];
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) s.kernel_ms = ms;
     if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]) == hipSuccess) s.total_ms = ms;

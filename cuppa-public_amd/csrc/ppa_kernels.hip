@@ -225,6 +225,11 @@ void generate_kernel(GenArgs a)
     // Wavefront-uniform claim window [wcur, wend) of node (or work) indices,
     // and for batches the window's graph.
     uint64_t wcur = 0, wend = 0, wlast = 0, fresh_base = 0;
+    // Single graphs: the nodes assigned in one pass may span the end of one
+    // claim window and the start of the next; fresh node i < fresh_split is
+    // fresh_base + i, later ones fresh_base2 + (i - fresh_split).
+    uint64_t fresh_base2 = 0;
+    uint32_t fresh_split = kWave;
     bool wdone = false;
     int32_t *wG = a.G;
     uint64_t wT = a.T, wtoff = 0;
@@ -297,9 +302,31 @@ void generate_kernel(GenArgs a)
             if (!wdone) {
                 const uint64_t avail = wend - wcur;
                 const uint32_t rank = rank_below(idle_mask);
+                const uint64_t idle = (uint64_t)__popcll(idle_mask);
                 fresh_base = wcur + wtoff;
-                if (!active && rank < avail) {
-                    t = wcur + rank + wtoff;
+                fresh_split = kWave;
+                // Single graphs: a window that cannot feed every idle lane is
+                // topped up with the next chunk in the same pass (claims stay
+                // ascending); otherwise its last few nodes would take a pass
+                // of their own with the other idle lanes doing nothing.
+                uint64_t avail2 = 0;
+                if (!BATCH && avail < idle) {
+                    const uint32_t size = wlast < a.early_nodes ? a.early_chunk : a.chunk;
+                    uint64_t base = 0;
+                    if (lane == 0) base = atomicAdd(&a.scratch->claim, (unsigned long long)size);
+                    base = uniform64(base);
+                    ++s_claims;
+                    wlast = base;
+                    if (base >= a.n) {
+                        wdone = true;  // the old window's remainder is assigned below
+                    } else {
+                        fresh_split = (uint32_t)avail;
+                        fresh_base2 = base;
+                        avail2 = base + size < a.n ? size : a.n - base;
+                    }
+                }
+                if (!active && rank < avail + avail2) {
+                    t = rank < avail ? wcur + rank + wtoff : fresh_base2 + (rank - avail);
                     active = true;
                     fresh = true;
                     progressed = true;
@@ -314,15 +341,20 @@ void generate_kernel(GenArgs a)
                     }
                     early = t < a.early_nodes;
                 }
-                const uint64_t idle = (uint64_t)__popcll(idle_mask);
-                wcur += idle < avail ? idle : avail;
+                if (avail2 != 0) {
+                    wcur = fresh_base2 + (idle - avail < avail2 ? idle - avail : avail2);
+                    wend = fresh_base2 + avail2;
+                } else {
+                    wcur += idle < avail ? idle : avail;
+                }
             }
         }
         if (wdone && __ballot(active) == 0) break;
 
         // ---- attempt 0 of the nodes assigned this pass, over the whole wavefront ----
-        // The f lanes assigned this pass hold the consecutive nodes
-        // fresh_base + 0 .. f-1 of one claim window. Lane a % 64 computes pair
+        // The f lanes assigned this pass hold, in rank order, the consecutive
+        // nodes fresh_base + 0 .. of one claim window (single graphs: then
+        // fresh_base2 + 0 .. of the next). Lane a % 64 computes pair
         // a = i*d + kk (node i, slot kk) in pass a / 64 -- ceil(f*d / 64)
         // Philox passes where per-lane draws would cost d. A copy read goes
         // straight into word a of this wavefront's LDS stage (global_load_lds:
@@ -343,7 +375,8 @@ void generate_kernel(GenArgs a)
                 if (ai < total) {
                     const uint32_t i = BATCH ? (ai * magic) >> 16 : ai / (uint32_t)DMAX;
                     const uint32_t kk = ai - i * fd;
-                    const uint64_t tt = fresh_base + i;
+                    const uint64_t tt =
+                        BATCH || i < fresh_split ? fresh_base + i : fresh_base2 + (i - fresh_split);
                     const Attempt at = attempt(wk0, wk1, wT, tt, kk, 0u, tt * fd);
                     if (at.direct) {
                         info[ai] = ~(int32_t)(BATCH ? div_small(at.pair, fd, winv) : at.pair / (uint64_t)DMAX);
@@ -683,12 +716,12 @@ void generate_kernel(GenArgs a)
     const uint64_t pmiss = wave_sum64(c_pmiss), polls = wave_sum64(c_polls), stalls = wave_sum64(c_stalls);
     if (lane == 0) {
         unsigned long long *st = a.scratch->stats;
-        const uint64_t v[10] = {s_nodes, slots, draws, s_retries, pmiss, polls, stalls, s_claims, s_sleeps,
-                                s_staged};
-        const int idx[10] = {kStatNodes,  kStatSlots,  kStatDraws,  kStatRetries, kStatPrefetchMisses,
-                             kStatPolls,  kStatStalls, kStatClaims, kStatSleeps,  kStatStagedRows};
+        const uint64_t v[11] = {s_nodes, slots, draws, s_retries, pmiss, polls, stalls, s_claims, s_sleeps,
+                                s_staged, iter};
+        const int idx[11] = {kStatNodes, kStatSlots,  kStatDraws,  kStatRetries, kStatPrefetchMisses, kStatPolls,
+                             kStatStalls, kStatClaims, kStatSleeps, kStatStagedRows, kStatPasses};
 #pragma unroll
-        for (int i = 0; i < 10; ++i)
+        for (int i = 0; i < 11; ++i)
             if (v[i]) atomicAdd(&st[idx[i]], (unsigned long long)v[i]);
     }
 }

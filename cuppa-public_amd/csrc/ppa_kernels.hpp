@@ -17,6 +17,7 @@ enum StatIndex : int {
     kStatClaims,
     kStatSleeps,
     kStatStagedRows,
+    kStatPasses,
     kStatCount
 };
 

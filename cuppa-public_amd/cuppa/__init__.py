@@ -61,7 +61,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("nodes", ctypes.c_uint64), ("slots", ctypes.c_uint64), ("draws", ctypes.c_uint64),
                 ("retries", ctypes.c_uint64), ("prefetch_misses", ctypes.c_uint64),
                 ("polls", ctypes.c_uint64), ("stalls", ctypes.c_uint64), ("claims", ctypes.c_uint64),
-                ("sleeps", ctypes.c_uint64), ("staged_rows", ctypes.c_uint64), ("grid_blocks", ctypes.c_uint64),
+                ("sleeps", ctypes.c_uint64), ("staged_rows", ctypes.c_uint64), ("passes", ctypes.c_uint64),
+                ("grid_blocks", ctypes.c_uint64),
                 ("block_threads", ctypes.c_uint64), ("kernel_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double)]
 

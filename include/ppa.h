@@ -116,6 +116,8 @@ typedef struct ppa_stats {
     uint64_t claims;         /* work-claim atomics */
     uint64_t sleeps;         /* wavefront back-off sleeps */
     uint64_t staged_rows;    /* rows published through the coalesced LDS path */
+    uint64_t passes;         /* loop passes, summed over wavefronts (nodes / passes = lanes
+                                refilled per pass) */
     uint64_t grid_blocks;    /* persistent workgroups launched */
     uint64_t block_threads;
     double kernel_ms;        /* device time of the generation kernel */
