@@ -30,7 +30,8 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kPerThread = 16;
-constexpr int kTile = kThreads * kPerThread;  // slots a workgroup orders in LDS at once
+constexpr int kTile = kThreads * kPerThread;  // slots a count workgroup reads at once
+constexpr int kScatterThreads = 512;           // scatter workgroups order 8192 slots at once
 constexpr uint32_t kMaxBins = 512;
 constexpr uint32_t kLeafBits = 14;
 constexpr uint32_t kLeafNodes = 1u << kLeafBits;
@@ -205,6 +206,11 @@ __global__ __launch_bounds__(kMaxBins) void plan_level2_kernel(const uint64_t *o
     }
 }
 
+// Bin 0 of a partition holds the smallest node ids, which in a
+// preferential-attachment graph are named by far the most slots (a third of
+// them at the first level): its updates are counted per lane, or ranked per
+// wavefront, instead of contending for one LDS word.
+
 // counts[(segment, bin, workgroup)] = slots of the workgroup's range in bin;
 // with leaf_size, also the slots of every (segment, bin) leaf.
 __global__ __launch_bounds__(kThreads) void part_count_kernel(const int32_t *in, PartPlan pl, uint64_t n,
@@ -215,18 +221,35 @@ __global__ __launch_bounds__(kThreads) void part_count_kernel(const int32_t *in,
     const BlockRange r = block_range(pl, blockIdx.x);
     for (uint32_t i = threadIdx.x; i < pl.nbins; i += kThreads) hist[i] = 0;
     __syncthreads();
-    // kPerThread reads in flight per thread, then their LDS updates
+    // kPerThread reads in flight per thread; the next tile's are issued
+    // before this tile's LDS updates
+    uint32_t hot = 0;  // this lane's slots in bin 0
+    int32_t v[kPerThread];
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        const uint64_t i = r.lo + (uint64_t)j * kThreads + threadIdx.x;
+        v[j] = i < r.hi ? in[i] : -1;
+    }
     for (uint64_t t0 = r.lo; t0 < r.hi; t0 += kTile) {
-        int32_t v[kPerThread];
+        int32_t nv[kPerThread];
 #pragma unroll
         for (int j = 0; j < kPerThread; ++j) {
-            const uint64_t i = t0 + (uint64_t)j * kThreads + threadIdx.x;
-            v[j] = i < r.hi ? in[i] : -1;
+            const uint64_t i = t0 + kTile + (uint64_t)j * kThreads + threadIdx.x;
+            nv[j] = i < r.hi ? in[i] : -1;
         }
 #pragma unroll
-        for (int j = 0; j < kPerThread; ++j)
-            if (valid(v[j], n)) atomicAdd(&hist[((uint32_t)v[j] >> pl.shift) & pl.mask], 1u);
+        for (int j = 0; j < kPerThread; ++j) {
+            if (valid(v[j], n)) {
+                const uint32_t bin = ((uint32_t)v[j] >> pl.shift) & pl.mask;
+                if (bin == 0) ++hot;
+                else atomicAdd(&hist[bin], 1u);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kPerThread; ++j) v[j] = nv[j];
     }
+    hot = wave_sum(hot);
+    if ((threadIdx.x & 63u) == 0 && hot) atomicAdd(&hist[0], hot);
     __syncthreads();
     for (uint32_t bin = threadIdx.x; bin < pl.nbins; bin += kThreads) {
         const uint32_t c = hist[bin];
@@ -286,49 +309,84 @@ __global__ __launch_bounds__(kThreads) void scan_apply_kernel(const uint64_t *co
 
 // Moves every valid slot of the workgroup's range to its bin's run of the
 // output; tile by tile, slots are grouped by bin in LDS before the stores.
-__global__ __launch_bounds__(kThreads) void part_scatter_kernel(const int32_t *in, int32_t *out, PartPlan pl,
-                                                                uint64_t n, const uint64_t *offsets)
+// TH threads place TH * kPerThread slots per tile: at 512 threads a bin's run
+// in a tile averages 32 slots at 256 bins, so most of a tile's stores fill
+// whole 64-byte lines.
+template <int TH>
+__global__ __launch_bounds__(TH) void part_scatter_kernel(const int32_t *in, int32_t *out, PartPlan pl, uint64_t n,
+                                                          const uint64_t *offsets)
 {
+    constexpr int kT = TH * kPerThread;
+    constexpr uint32_t kBinsPerThread = (kMaxBins + TH - 1) / TH;
     __shared__ uint64_t cur[kMaxBins];      // next output slot of each bin
     __shared__ uint32_t thist[kMaxBins];    // slots of each bin in this tile
     __shared__ uint32_t tstart[kMaxBins];   // their exclusive scan
-    __shared__ int32_t tile[kTile];
-    __shared__ uint64_t red[kThreads / 64];
+    __shared__ int32_t tile[kT];
+    __shared__ uint64_t red[TH / 64];
     if (blockIdx.x >= pl.seg_b0[pl.nseg]) return;
     const BlockRange r = block_range(pl, blockIdx.x);
-    for (uint32_t bin = threadIdx.x; bin < pl.nbins; bin += kThreads) {
+    for (uint32_t bin = threadIdx.x; bin < pl.nbins; bin += TH) {
         cur[bin] = offsets[r.count_base + (uint64_t)bin * r.nb + r.rank];
         thist[bin] = 0;
     }
     __syncthreads();
-    for (uint64_t t0 = r.lo; t0 < r.hi; t0 += kTile) {
-        int32_t v[kPerThread];
+    int32_t v[kPerThread];
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        const uint64_t i = r.lo + (uint64_t)j * TH + threadIdx.x;
+        v[j] = i < r.hi ? in[i] : -1;
+    }
+    for (uint64_t t0 = r.lo; t0 < r.hi; t0 += kT) {
         uint32_t rank[kPerThread];
 #pragma unroll
         for (int j = 0; j < kPerThread; ++j) {
-            const uint64_t i = t0 + (uint64_t)j * kThreads + threadIdx.x;
-            v[j] = i < r.hi ? in[i] : -1;
-            rank[j] = valid(v[j], n) ? atomicAdd(&thist[((uint32_t)v[j] >> pl.shift) & pl.mask], 1u) : 0u;
+            const bool ok = valid(v[j], n);
+            const uint32_t bin = ((uint32_t)v[j] >> pl.shift) & pl.mask;
+            // bin 0: one LDS atomic per wavefront, ranks by lane order
+            const uint64_t m0 = __ballot(ok && bin == 0);
+            uint32_t base0 = 0;
+            if (m0 != 0) {
+                const uint32_t leader = (uint32_t)__builtin_ctzll(m0);
+                if ((threadIdx.x & 63u) == leader) base0 = atomicAdd(&thist[0], (uint32_t)__popcll(m0));
+                base0 = (uint32_t)__builtin_amdgcn_readlane((int)base0, (int)leader);
+            }
+            rank[j] = !ok ? 0u : bin == 0 ? base0 + rank_below(m0) : atomicAdd(&thist[bin], 1u);
         }
         __syncthreads();
-        const uint32_t b0 = 2 * threadIdx.x, b1 = b0 + 1;
-        const uint32_t c0 = b0 < pl.nbins ? thist[b0] : 0u, c1 = b1 < pl.nbins ? thist[b1] : 0u;
+        // bins [kBinsPerThread * tid, +kBinsPerThread) per thread
+        uint32_t c[kBinsPerThread], csum = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kBinsPerThread; ++q) {
+            const uint32_t b = kBinsPerThread * threadIdx.x + q;
+            c[q] = b < pl.nbins ? thist[b] : 0u;
+            csum += c[q];
+        }
         uint64_t tot = 0;
-        const uint32_t ex = (uint32_t)block_exclusive_scan<kThreads>(c0 + c1, &tot, red);
-        if (b0 < pl.nbins) tstart[b0] = ex;
-        if (b1 < pl.nbins) tstart[b1] = ex + c0;
+        uint32_t ex = (uint32_t)block_exclusive_scan<TH>(csum, &tot, red);
+#pragma unroll
+        for (uint32_t q = 0; q < kBinsPerThread; ++q) {
+            const uint32_t b = kBinsPerThread * threadIdx.x + q;
+            if (b < pl.nbins) tstart[b] = ex;
+            ex += c[q];
+        }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kPerThread; ++j)
             if (valid(v[j], n)) tile[tstart[((uint32_t)v[j] >> pl.shift) & pl.mask] + rank[j]] = v[j];
+        // the next tile's reads are in flight while this one is stored
+#pragma unroll
+        for (int j = 0; j < kPerThread; ++j) {
+            const uint64_t i = t0 + kT + (uint64_t)j * TH + threadIdx.x;
+            v[j] = i < r.hi ? in[i] : -1;
+        }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < (uint32_t)tot; i += kThreads) {
+        for (uint32_t i = threadIdx.x; i < (uint32_t)tot; i += TH) {
             const int32_t w = tile[i];
             const uint32_t bin = ((uint32_t)w >> pl.shift) & pl.mask;
             out[cur[bin] + (i - tstart[bin])] = w;
         }
         __syncthreads();
-        for (uint32_t bin = threadIdx.x; bin < pl.nbins; bin += kThreads) {
+        for (uint32_t bin = threadIdx.x; bin < pl.nbins; bin += TH) {
             cur[bin] += thist[bin];
             thist[bin] = 0;
         }
@@ -436,7 +494,7 @@ hipError_t partition(const int32_t *in, int32_t *out, const PartPlan &pl, uint32
     hipLaunchKernelGGL(part_count_kernel, dim3(grid), dim3(kThreads), 0, stream, in, pl, n, w.counts, leaf_size);
     hipError_t e = scan(w.counts, count_len, w, stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(part_scatter_kernel, dim3(grid), dim3(kThreads), 0, stream, in, out, pl, n,
+    hipLaunchKernelGGL(part_scatter_kernel<kScatterThreads>, dim3(grid), dim3(kScatterThreads), 0, stream, in, out, pl, n,
                        (const uint64_t *)w.offsets);
     return hipGetLastError();
 }
