@@ -506,13 +506,15 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // keeping the order of equal digits; after ceil(bits(n-1) / 8) passes the
 // sources of each target are in slot order, which is ascending source order.
 // Ranks are exact without LDS atomics: a wavefront owns 1024 consecutive
-// slots of a 4096-slot tile and walks them 64 at a time; lanes with equal
-// digits are matched with 8 ballots and counted from a wavefront-private
-// running count, and the four wavefronts' counts are combined per digit.
+// slots of a tile (8192 slots for the 8 wavefronts of a workgroup) and walks
+// them 64 at a time; lanes with equal digits are matched with 8 ballots and
+// counted from a wavefront-private running count, and the wavefronts' counts
+// are combined per digit. A digit's run in a tile averages 32 pairs, so most
+// of its stores fill whole 64-byte lines.
 constexpr uint32_t kRadixBits = 8;
 constexpr uint32_t kRadixBins = 1u << kRadixBits;
-constexpr uint32_t kWaves = kThreads / 64;
-constexpr uint32_t kWaveShare = kTile / kWaves;   // slots per wavefront per tile
+constexpr uint32_t kLsdThreads = 512;
+constexpr uint32_t kWaveShare = 1024;             // slots per wavefront per tile
 constexpr uint32_t kSteps = kWaveShare / 64;      // 64-slot steps per wavefront
 
 // Lanes among `act` whose digit equals this lane's; whole wavefront.
@@ -543,21 +545,26 @@ __global__ void plan_total_kernel(uint32_t *seg_b0, uint64_t *seg_start, uint32_
 // One stable pass. vin == nullptr: the sources are the slot indices' rows,
 // (first + i) / d (first pass, reading the targets array itself). kout may be
 // null (no later pass needs the keys).
-__global__ __launch_bounds__(kThreads) void lsd_scatter_kernel(const int32_t *kin, const int32_t *vin, int32_t *kout,
-                                                               int32_t *vout, PartPlan pl, uint64_t n, uint32_t d,
-                                                               double inv_d, uint64_t first, const uint64_t *offsets)
+template <uint32_t TH>
+__global__ __launch_bounds__(TH) void lsd_scatter_kernel(const int32_t *kin, const int32_t *vin, int32_t *kout,
+                                                         int32_t *vout, PartPlan pl, uint64_t n, uint32_t d,
+                                                         double inv_d, uint64_t first, const uint64_t *offsets)
 {
+    constexpr uint32_t kWaves = TH / 64;
+    constexpr uint32_t kT = kWaves * kWaveShare;
+    static_assert(TH >= kRadixBins, "one thread per digit");
     __shared__ uint64_t cur[kRadixBins];            // next output position per digit
     __shared__ uint32_t wrun[kWaves][kRadixBins];   // per-wavefront counts, then their prefix over wavefronts
     __shared__ uint32_t tstart[kRadixBins];         // tile start of each digit
-    __shared__ int32_t tk[kTile], tv[kTile];
+    __shared__ int32_t tk[kT], tv[kT];
     __shared__ uint64_t red[kWaves];
     if (blockIdx.x >= pl.seg_b0[pl.nseg]) return;
     const BlockRange r = block_range(pl, blockIdx.x);
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    cur[threadIdx.x] = offsets[r.count_base + (uint64_t)threadIdx.x * r.nb + r.rank];
+    const bool digit_thread = threadIdx.x < kRadixBins;
+    if (digit_thread) cur[threadIdx.x] = offsets[r.count_base + (uint64_t)threadIdx.x * r.nb + r.rank];
     __syncthreads();
-    for (uint64_t t0 = r.lo; t0 < r.hi; t0 += kTile) {
+    for (uint64_t t0 = r.lo; t0 < r.hi; t0 += kT) {
         for (uint32_t b = lane; b < kRadixBins; b += 64) wrun[wave][b] = 0;
         int32_t k[kSteps], v[kSteps];
         uint32_t rk[kSteps];
@@ -582,14 +589,17 @@ __global__ __launch_bounds__(kThreads) void lsd_scatter_kernel(const int32_t *ki
         __syncthreads();
         // thread = digit: prefix of the wavefronts' counts, and the tile total
         uint32_t tot = 0;
+        if (digit_thread) {
 #pragma unroll
-        for (uint32_t w = 0; w < kWaves; ++w) {
-            const uint32_t c = wrun[w][threadIdx.x];
-            wrun[w][threadIdx.x] = tot;
-            tot += c;
+            for (uint32_t w = 0; w < kWaves; ++w) {
+                const uint32_t c = wrun[w][threadIdx.x];
+                wrun[w][threadIdx.x] = tot;
+                tot += c;
+            }
         }
         uint64_t total = 0;
-        tstart[threadIdx.x] = (uint32_t)block_exclusive_scan<kThreads>(tot, &total, red);
+        const uint32_t ts = (uint32_t)block_exclusive_scan<TH>(tot, &total, red);
+        if (digit_thread) tstart[threadIdx.x] = ts;
         __syncthreads();
 #pragma unroll
         for (uint32_t j = 0; j < kSteps; ++j) {
@@ -601,7 +611,7 @@ __global__ __launch_bounds__(kThreads) void lsd_scatter_kernel(const int32_t *ki
             }
         }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < (uint32_t)total; i += kThreads) {
+        for (uint32_t i = threadIdx.x; i < (uint32_t)total; i += TH) {
             const int32_t kk = tk[i];
             const uint32_t dig = ((uint32_t)kk >> pl.shift) & (kRadixBins - 1);
             const uint64_t o = cur[dig] + (i - tstart[dig]);
@@ -609,7 +619,7 @@ __global__ __launch_bounds__(kThreads) void lsd_scatter_kernel(const int32_t *ki
             vout[o] = tv[i];
         }
         __syncthreads();
-        cur[threadIdx.x] += tot;
+        if (digit_thread) cur[threadIdx.x] += tot;
         __syncthreads();
     }
 }
@@ -799,7 +809,8 @@ hipError_t launch_in_csr(const int32_t *G, uint64_t n, uint32_t d, int64_t *in_p
         hipLaunchKernelGGL(part_count_kernel, dim3(g.blocks), dim3(kThreads), 0, stream, kin, pl, n, w.counts,
                            (uint64_t *)nullptr);
         if ((e = scan(w.counts, len, w, stream)) != hipSuccess) return e;
-        hipLaunchKernelGGL(lsd_scatter_kernel, dim3(g.blocks), dim3(kThreads), 0, stream, kin, vin, kout, vout, pl, n,
+        hipLaunchKernelGGL(lsd_scatter_kernel<kLsdThreads>, dim3(g.blocks), dim3(kLsdThreads), 0, stream, kin, vin, kout,
+                           vout, pl, n,
                            d, inv_d, 0ull, (const uint64_t *)w.offsets);
         if (pass == 0)  // later passes cover the valid pairs only
             hipLaunchKernelGGL(plan_total_kernel, dim3(1), dim3(64), 0, stream, seg_b0, seg_start, g.blocks,
