@@ -62,6 +62,53 @@ int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *ou
     return PPA_OK;
 }
 
+uint64_t single_start(const ppa_ctx *ctx, const ppa_graph_desc &desc)
+{
+    return prefix_start(desc.n, desc.d, coin_threshold(desc.model, desc.p_threshold), ctx->tuning.prefix_nodes);
+}
+
+hipError_t enqueue_single(const ppa_ctx *ctx, const ppa_graph_desc &desc, int32_t *G, int64_t *row_ptr,
+                          const LaunchShape &shape, DeviceScratch *scratch, hipStream_t stream, hipEvent_t after_fill)
+{
+    const uint64_t n = desc.n;
+    const uint32_t d = desc.d;
+    const uint64_t start = single_start(ctx, desc);
+    const uint64_t T = coin_threshold(desc.model, desc.p_threshold);
+    hipError_t e = launch_reset_scratch(scratch, start, stream);
+    if (e != hipSuccess) return e;
+    if ((e = launch_fill(FillArgs{G, row_ptr, n, d, desc.model}, stream)) != hipSuccess) return e;
+    if (after_fill && (e = hipEventRecord(after_fill, stream)) != hipSuccess) return e;
+    if (start > (uint64_t)d + 1) {
+        PrefixArgs pa{};
+        pa.G = G;
+        pa.T = T;
+        pa.key0 = (uint32_t)desc.seed;
+        pa.key1 = (uint32_t)(desc.seed >> 32);
+        pa.model = desc.model;
+        pa.prefix = start;
+        pa.watchdog_ticks = shape.watchdog_ticks;
+        pa.scratch = scratch;
+        if ((e = launch_prefix(d, pa, stream)) != hipSuccess) return e;
+    }
+    if (n > start) {
+        GenArgs ga{};
+        ga.G = G;
+        ga.n = n;
+        ga.T = T;
+        ga.key0 = (uint32_t)desc.seed;
+        ga.key1 = (uint32_t)(desc.seed >> 32);
+        ga.chunk = shape.chunk;
+        ga.early_chunk = kEarlyChunk;
+        ga.early_nodes = shape.early_nodes;
+        ga.watchdog_ticks = shape.watchdog_ticks;
+        ga.sleep_cap = shape.sleep_cap;
+        ga.wide_mask = shape.wide_mask;
+        ga.scratch = scratch;
+        if ((e = launch_generate(d, ga, shape.blocks, shape.threads, stream)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 int ensure_work(ppa_ctx *ctx, size_t need, hipStream_t stream)
 {
     if (ctx->work_bytes >= need) return PPA_OK;
@@ -226,32 +273,14 @@ int ppa_generate_async(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targ
     const uint64_t n = desc->n;
     const uint32_t d = desc->d;
     LaunchShape shape;
-    if ((rc = resolve_shape(ctx, d, n - d - 1, &shape)) != PPA_OK) return rc;
+    const uint64_t start = single_start(ctx, *desc);
+    if ((rc = resolve_shape(ctx, d, n > start ? n - start : 1, &shape)) != PPA_OK) return rc;
 
     ctx->last = ppa_stats{};
     ctx->last.grid_blocks = shape.blocks;
     ctx->last.block_threads = shape.threads;
     PPA_HIP(hipEventRecord(ctx->ev[0], stream));
-    PPA_HIP(launch_reset_scratch(ctx->scratch, (uint64_t)d + 1, stream));
-    FillArgs fa{d_targets, d_row_ptr, n, d, desc->model};
-    PPA_HIP(launch_fill(fa, stream));
-    PPA_HIP(hipEventRecord(ctx->ev[1], stream));
-    if (n > (uint64_t)d + 1) {
-        GenArgs ga{};
-        ga.G = d_targets;
-        ga.n = n;
-        ga.T = coin_threshold(desc->model, desc->p_threshold);
-        ga.key0 = (uint32_t)desc->seed;
-        ga.key1 = (uint32_t)(desc->seed >> 32);
-        ga.chunk = shape.chunk;
-        ga.early_chunk = kEarlyChunk;
-        ga.early_nodes = shape.early_nodes;
-        ga.watchdog_ticks = shape.watchdog_ticks;
-        ga.sleep_cap = shape.sleep_cap;
-        ga.wide_mask = shape.wide_mask;
-        ga.scratch = ctx->scratch;
-        PPA_HIP(launch_generate(d, ga, shape.blocks, shape.threads, stream));
-    }
+    PPA_HIP(enqueue_single(ctx, *desc, d_targets, d_row_ptr, shape, ctx->scratch, stream, ctx->ev[1]));
     PPA_HIP(hipEventRecord(ctx->ev[2], stream));
     ctx->pending = true;
     ctx->expect_nodes = n - d - 1;

@@ -37,10 +37,10 @@ namespace ppa {
 size_t batch_tables_bytes(uint64_t count)
 {
     return count * sizeof(BatchGraph) + (2 * (count + 1) + (count + 1) + 2 * count) * sizeof(uint64_t) +
-           2 * count * sizeof(uint32_t);
+           3 * count * sizeof(uint32_t);
 }
 
-BatchGraph graph_record(const ppa_graph_desc &dsc)
+BatchGraph graph_record(const ppa_graph_desc &dsc, uint32_t prefix_cap)
 {
     BatchGraph gr;
     std::memset(&gr, 0, sizeof(gr));
@@ -51,6 +51,7 @@ BatchGraph graph_record(const ppa_graph_desc &dsc)
     gr.d = dsc.d;
     gr.model = dsc.model;
     gr.inv_d = 1.0 / (double)dsc.d;
+    gr.start = prefix_start(dsc.n, dsc.d, gr.T, prefix_cap);
     return gr;
 }
 
@@ -61,6 +62,7 @@ struct TableView {
     uint64_t *slot, *row, *seg_base, *seg_round, *seg_active;
     uint32_t *order;
     uint32_t *done;  // finished nodes per graph
+    uint32_t *pidx;  // graphs with an LDS-resident prefix
 };
 
 TableView view_of(char *base, uint64_t count, uint64_t nseg)
@@ -74,6 +76,7 @@ TableView view_of(char *base, uint64_t count, uint64_t nseg)
     v.seg_active = v.seg_round + nseg;
     v.order = (uint32_t *)(v.seg_active + nseg);
     v.done = v.order + count;
+    v.pidx = v.done + count;
     return v;
 }
 
@@ -86,9 +89,13 @@ void build_batch_tables(const std::vector<BatchGraph> &graphs, uint64_t chunk, B
     std::vector<uint32_t> order(count);
     out->work = 0;
     out->any_row_ptr = false;
+    std::vector<uint32_t> pidx;
+    out->nodes = 0;
     for (uint64_t i = 0; i < count; ++i) {
-        const uint64_t w = graphs[i].n - graphs[i].d - 1;
+        const uint64_t w = graphs[i].n - graphs[i].start;
         out->work += w;
+        out->nodes += graphs[i].n - graphs[i].d - 1;
+        if (graphs[i].start > (uint64_t)graphs[i].d + 1) pidx.push_back((uint32_t)i);
         chunks[i] = (w + chunk - 1) / chunk;
         order[i] = (uint32_t)i;
         out->any_row_ptr |= graphs[i].row_ptr != nullptr;
@@ -110,7 +117,7 @@ void build_batch_tables(const std::vector<BatchGraph> &graphs, uint64_t chunk, B
     out->count = count;
     out->nseg = nseg;
     out->bytes.assign(count * sizeof(BatchGraph) + (2 * (count + 1) + (nseg + 1) + 2 * nseg) * sizeof(uint64_t) +
-                          2 * count * sizeof(uint32_t),
+                          3 * count * sizeof(uint32_t),
                       0);
     TableView v = view_of(out->bytes.data(), count, nseg);
     std::memcpy(v.graphs, graphs.data(), count * sizeof(BatchGraph));
@@ -125,6 +132,8 @@ void build_batch_tables(const std::vector<BatchGraph> &graphs, uint64_t chunk, B
         std::memcpy(v.seg_active, seg_active.data(), nseg * sizeof(uint64_t));
     }
     std::memcpy(v.order, order.data(), count * sizeof(uint32_t));
+    if (!pidx.empty()) std::memcpy(v.pidx, pidx.data(), pidx.size() * sizeof(uint32_t));
+    out->nprefix = pidx.size();
     out->total_slots = v.slot[count];
     out->total_rows = out->any_row_ptr ? v.row[count] : 0;
 }
@@ -137,6 +146,15 @@ hipError_t enqueue_batch(void *d_tables, const BatchTables &t, const LaunchShape
     if (e != hipSuccess) return e;
     BatchFillArgs fa{v.graphs, v.slot, v.row, t.count, v.done};
     if ((e = launch_fill_batch(fa, t.total_slots, t.total_rows, stream)) != hipSuccess) return e;
+    if (t.nprefix != 0) {
+        PrefixArgs pa{};
+        pa.graphs = v.graphs;
+        pa.pidx = v.pidx;
+        pa.watchdog_ticks = shape.watchdog_ticks;
+        pa.scratch = scratch;
+        pa.done_flag = done_flag;
+        if ((e = launch_prefix_batch(pa, (uint32_t)t.nprefix, stream)) != hipSuccess) return e;
+    }
     if (t.work == 0) return hipSuccess;
     GenArgs ga{};
     ga.graphs = v.graphs;
@@ -199,7 +217,7 @@ int ppa_batch_create(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, 
         if (rc) return rc;
         treg.emplace_back(descs[i].targets_offset, descs[i].n * descs[i].d);
         rreg.emplace_back(descs[i].row_ptr_offset, descs[i].n + 1);
-        graphs[i] = graph_record(descs[i]);
+        graphs[i] = graph_record(descs[i], ctx->tuning.prefix_nodes);
         graphs[i].G = d_targets + descs[i].targets_offset;
         graphs[i].row_ptr = d_row_ptr ? d_row_ptr + descs[i].row_ptr_offset : nullptr;
         graphs[i].targets_offset = descs[i].targets_offset;
@@ -213,7 +231,7 @@ int ppa_batch_create(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, 
     b->ctx = ctx;
     b->count = count;
     uint64_t work = 0;
-    for (const auto &gr : graphs) work += gr.n - gr.d - 1;
+    for (const auto &gr : graphs) work += gr.n - gr.start;
     int rc = resolve_shape(ctx, 0, work, &b->shape);
     if (rc) {
         destroy(b);
@@ -221,7 +239,7 @@ int ppa_batch_create(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, 
     }
     BatchTables tables;
     build_batch_tables(graphs, b->shape.chunk, &tables);
-    b->expect_nodes = tables.work;
+    b->expect_nodes = tables.nodes;
     if (hipMalloc(&b->d_tables, tables.bytes.size()) != hipSuccess) {
         destroy(b);
         return PPA_ERR_NOMEM;

@@ -70,6 +70,15 @@ struct LaunchShape {
 // Grid, workgroup and claim sizes for `work` generated nodes with out-degree
 // d (d = 0: the batch kernel).
 int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *out);
+// First node the persistent generator claims for a single graph: after the
+// LDS-resident prefix, or d + 1 without it.
+uint64_t single_start(const ppa_ctx *ctx, const ppa_graph_desc &desc);
+// Reset, fill, prefix and persistent generator of one graph on `stream`;
+// `shape` was resolved for the work n - single_start(ctx, n, d). after_fill
+// (nullable) is recorded between the fill and the generation.
+hipError_t enqueue_single(const ppa_ctx *ctx, const ppa_graph_desc &desc, int32_t *G, int64_t *row_ptr,
+                          const LaunchShape &shape, DeviceScratch *scratch, hipStream_t stream,
+                          hipEvent_t after_fill = nullptr);
 
 // Grows the context's work area to `need` bytes (waiting for `stream` before
 // freeing the old one); PPA_ERR_NOMEM if the device cannot provide it.
@@ -82,7 +91,9 @@ struct BatchTables {
     std::vector<char> bytes;
     uint64_t count = 0;
     uint64_t nseg = 0;
-    uint64_t work = 0;         // generated nodes
+    uint64_t work = 0;         // nodes for the persistent generator
+    uint64_t nodes = 0;        // generated nodes, the prefixes' included
+    uint64_t nprefix = 0;      // graphs with an LDS-resident prefix
     uint64_t total_slots = 0;  // sum of n*d
     uint64_t total_rows = 0;   // sum of n+1 over graphs with a row_ptr
     bool any_row_ptr = false;
@@ -97,7 +108,8 @@ void build_batch_tables(const std::vector<BatchGraph> &graphs, uint64_t chunk, B
 hipError_t enqueue_batch(void *d_tables, const BatchTables &t, const LaunchShape &shape, DeviceScratch *scratch,
                          hipStream_t stream, uint32_t *done_flag = nullptr);
 // A descriptor -> the kernel's graph record (G, row_ptr and offsets set by the caller).
-BatchGraph graph_record(const ppa_graph_desc &desc);
+// prefix_cap: the tuning's prefix_nodes (see prefix_start).
+BatchGraph graph_record(const ppa_graph_desc &desc, uint32_t prefix_cap);
 // Waits for the stream and turns the device scratch into ctx->last.
 int finish_launch(ppa_ctx *ctx, hipStream_t stream);
 bool tuning_valid(const ppa_tuning &t);

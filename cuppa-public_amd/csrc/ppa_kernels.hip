@@ -281,7 +281,7 @@ void generate_kernel(GenArgs a)
                         const uint64_t round = a.seg_round[sg] + off / act;
                         wgid = a.order[off % act];
                         const BatchGraph &gr = a.graphs[wgid];
-                        const uint64_t work = gr.n - gr.d - 1;
+                        const uint64_t work = gr.n - gr.start;
                         wcur = round * size;
                         wend = wcur + size < work ? wcur + size : work;
                         wG = gr.G;
@@ -290,7 +290,7 @@ void generate_kernel(GenArgs a)
                         wk1 = gr.key1;
                         wd = gr.d;
                         winv = gr.inv_d;
-                        wtoff = (uint64_t)gr.d + 1;
+                        wtoff = gr.start;
                     }
                 } else if (base >= a.n) {
                     wdone = true;
@@ -673,7 +673,7 @@ void generate_kernel(GenArgs a)
                         const uint32_t old = __hip_atomic_fetch_add(&a.done_nodes[g], c, __ATOMIC_RELAXED,
                                                                     __HIP_MEMORY_SCOPE_AGENT);
                         const BatchGraph &gr = a.graphs[g];
-                        if (a.done_flag && (uint64_t)old + c == gr.n - gr.d - 1) {
+                        if (a.done_flag && (uint64_t)old + c == gr.n - gr.start) {
                             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                             __hip_atomic_store(&a.done_flag[g], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
                         }
@@ -724,6 +724,132 @@ void generate_kernel(GenArgs a)
         for (int i = 0; i < 11; ++i)
             if (v[i]) atomicAdd(&st[idx[i]], (unsigned long long)v[i]);
     }
+}
+
+// ---- LDS-resident prefix ----------------------------------------------------
+// A small graph's run time is its longest dependency chain times the cost of
+// one dependent step, and most of that chain lies among the first nodes,
+// whose copy sources are nearly all still being generated. One workgroup
+// generates nodes d+1 .. prefix-1 with the graph held in LDS, where a
+// dependent step costs an LDS read instead of an HBM round trip, then writes
+// the rows out; the persistent generator starts its claims at `prefix`.
+// Same draws, same order of acceptance, hence the same graph.
+//
+// Each of the 1024 lanes owns nodes d+1+lane, +1024, ... in ascending order.
+// A pass lets every lane advance until a copy source is still empty (-1);
+// passes are separated by workgroup barriers, which also make the slots
+// written in a pass visible to the next. The smallest unfinished node only
+// depends on finished slots, so every pass finishes at least one slot.
+constexpr int kPrefixThreads = (int)kPrefixLanes;
+
+template <int DMAX, bool BATCH>
+__global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
+{
+    __shared__ int32_t Gs[kPrefixSlots];
+    // a batch's graph, or the single graph of the arguments
+    uint32_t gi = 0;
+    if (BATCH) {
+        gi = a.pidx[blockIdx.x];
+        const BatchGraph &gr = a.graphs[gi];
+        a.G = gr.G;
+        a.T = gr.T;
+        a.key0 = gr.key0;
+        a.key1 = gr.key1;
+        a.model = gr.model;
+        a.prefix = gr.start;
+    }
+    const uint32_t d = BATCH ? a.graphs[gi].d : (uint32_t)DMAX;
+    const uint64_t P = a.prefix;
+    const uint32_t seed_slots = (d + 1) * d;
+    const uint32_t used = (uint32_t)P * d;
+    for (uint32_t s = threadIdx.x; s < used; s += kPrefixThreads)
+        Gs[s] = s < seed_slots ? seed_target(a.model, d, s / d, s % d) : kEmpty;
+    __syncthreads();
+
+    uint64_t t = (uint64_t)d + 1 + threadIdx.x;
+    bool active = t < P, waiting = false;
+    uint32_t k = 0, r = 0, src = 0;
+    int32_t val[DMAX];
+#pragma unroll
+    for (int i = 0; i < DMAX; ++i) val[i] = kEmpty;
+    uint32_t nodes = 0, draws = 0, retries = 0;
+    int fail = 0;
+    const uint64_t start_clock = wall_clock64();
+    for (uint32_t pass = 1;; ++pass) {
+        while (active) {
+            int32_t cand;
+            if (waiting) {
+                cand = __hip_atomic_load(&Gs[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (cand < 0) break;  // still empty: next pass
+                waiting = false;
+            } else {
+                const Attempt at = attempt(a.key0, a.key1, a.T, t, k, r, t * d);
+                ++draws;
+                if (at.direct) {
+                    cand = (int32_t)(at.pair / d);
+                } else {
+                    src = (uint32_t)at.pair;
+                    cand = __hip_atomic_load(&Gs[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (cand < 0) {
+                        waiting = true;
+                        break;
+                    }
+                }
+            }
+            bool dup = false;
+#pragma unroll
+            for (int i = 0; i < DMAX; ++i) dup |= (uint32_t)i < k && val[i] == cand;
+            if (dup) {
+                ++retries;
+                if (++r >= kMaxRetry) {
+                    fail = kErrRetry;
+                    active = false;
+                }
+                continue;
+            }
+#pragma unroll
+            for (int i = 0; i < DMAX; ++i)
+                if ((uint32_t)i == k) val[i] = cand;
+            __hip_atomic_store(&Gs[t * d + k], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            r = 0;
+            if (++k == d) {
+                ++nodes;
+                k = 0;
+                t += kPrefixThreads;
+                active = t < P;
+            }
+        }
+        if (__syncthreads_or(active) == 0) break;
+        if ((pass & 63u) == 0 && __syncthreads_or(wall_clock64() - start_clock > a.watchdog_ticks) != 0) {
+            fail = kErrTimeout;
+            break;
+        }
+    }
+    __syncthreads();
+    for (uint32_t s = seed_slots + threadIdx.x; s < used; s += kPrefixThreads) a.G[s] = Gs[s];
+    if (BATCH && a.done_flag && P == a.graphs[gi].n) {
+        // the prefix is the whole graph: no generator pass will raise its flag
+        __syncthreads();
+        if (threadIdx.x == 0 && fail == 0 && __hip_atomic_load(&a.scratch->status, __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+            __hip_atomic_store(&a.done_flag[gi], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+
+    const uint32_t wnodes = wave_sum(nodes), wdraws = wave_sum(draws), wretries = wave_sum(retries);
+    const int wfail = __ballot(fail != 0) != 0 ? 1 : 0;
+    if (fail != 0) atomicCAS(&a.scratch->status, 0, fail);
+    if (lane_id() == 0) {
+        unsigned long long *st = a.scratch->stats;
+        if (wnodes) {
+            atomicAdd(&st[kStatNodes], (unsigned long long)wnodes);
+            atomicAdd(&st[kStatSlots], (unsigned long long)wnodes * d);
+        }
+        if (wdraws) atomicAdd(&st[kStatDraws], (unsigned long long)wdraws);
+        if (wretries) atomicAdd(&st[kStatRetries], (unsigned long long)wretries);
+    }
+    (void)wfail;
 }
 
 // ---- degree pass ----------------------------------------------------------
@@ -811,6 +937,13 @@ hipError_t launch_generate_d(const GenArgs &a, uint32_t blocks, uint32_t threads
     return hipGetLastError();
 }
 
+template <int D>
+hipError_t launch_prefix_d(const PrefixArgs &a, hipStream_t stream)
+{
+    hipLaunchKernelGGL((prefix_kernel<D, false>), dim3(1), dim3(kPrefixThreads), 0, stream, a);
+    return hipGetLastError();
+}
+
 template <int D, bool BATCH>
 hipError_t occupancy_d(uint32_t threads, int *out)
 {
@@ -854,6 +987,23 @@ hipError_t launch_generate(uint32_t d, const GenArgs &a, uint32_t blocks, uint32
 #undef PPA_CASE
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_prefix(uint32_t d, const PrefixArgs &a, hipStream_t stream)
+{
+    switch (d) {
+#define PPA_CASE(DD) \
+    case DD: return launch_prefix_d<DD>(a, stream);
+        PPA_FOR_EACH_D(PPA_CASE)
+#undef PPA_CASE
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_prefix_batch(const PrefixArgs &a, uint32_t count, hipStream_t stream)
+{
+    hipLaunchKernelGGL((prefix_kernel<kBatchDMax, true>), dim3(count), dim3(kPrefixThreads), 0, stream, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_generate_batch(const GenArgs &a, uint32_t blocks, uint32_t threads, hipStream_t stream)

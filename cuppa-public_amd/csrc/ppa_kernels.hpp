@@ -41,8 +41,10 @@ struct BatchGraph {
     uint32_t d;
     uint32_t model;           // PPA_MODEL_* (seed rows)
     double inv_d;             // 1.0 / d
+    uint64_t start;           // first node the persistent generator claims (after the LDS prefix)
+    uint64_t pad;
 };
-static_assert(sizeof(BatchGraph) == 64, "BatchGraph layout");
+static_assert(sizeof(BatchGraph) == 80, "BatchGraph layout");
 
 struct GenArgs {
     // single graph
@@ -77,6 +79,54 @@ struct GenArgs {
     uint32_t *done_flag;
 };
 
+// The first nodes of a graph, generated in one workgroup's LDS: a single
+// graph (graphs == nullptr), or graph pidx[blockIdx.x] of a batch, whose
+// nodes [d+1, start) are generated.
+struct PrefixArgs {
+    int32_t *G;               // targets, row-major [n][d]
+    uint64_t T;
+    uint32_t key0, key1;
+    uint32_t model;           // PPA_MODEL_*: which seed rows
+    uint32_t pad;
+    uint64_t prefix;          // nodes [d+1, prefix) are generated; prefix * d <= kPrefixSlots
+    uint64_t watchdog_ticks;
+    DeviceScratch *scratch;
+    const BatchGraph *graphs;
+    const uint32_t *pidx;
+    uint32_t *done_flag;      // batch: per-graph host flag, raised here for a graph the prefix completes
+};
+
+// int32 slots of the LDS-resident prefix: 64 KB of LDS less the workgroup
+// reduction's 256 bytes. It holds prefix_nodes(n, d) nodes, over the
+// kPrefixLanes lanes of its one workgroup.
+constexpr uint32_t kPrefixSlots = 16320;
+constexpr uint32_t kPrefixLanes = 1024;
+inline uint64_t prefix_nodes(uint64_t n, uint32_t d)
+{
+    const uint64_t p = kPrefixSlots / d;
+    return n < p ? n : p;
+}
+// First node the persistent generator claims for a graph: the end of its
+// LDS-resident prefix, or d + 1 without one. prefix_cap is the tuning's
+// prefix_nodes: 0 = default (1024 + d + 1 nodes where a node has more than 4
+// copy draws on average, d(1-p) > 4, else none), 1 = none, N = at most N.
+// Measured at n = 1e5 .. 1e7: above 4 copies per node the prefix takes 6-22 %
+// off a call (d = 16, p = 0.1, n = 1e5: 5.45 -> 4.2 ms); at 4 or below it
+// costs up to 8 %, and more nodes per lane never help.
+inline uint64_t prefix_start(uint64_t n, uint32_t d, uint64_t T, uint32_t prefix_cap)
+{
+    if (prefix_cap == 1) return (uint64_t)d + 1;
+    uint64_t p = prefix_nodes(n, d);
+    if (prefix_cap != 0) {
+        p = p < prefix_cap ? p : prefix_cap;
+    } else {
+        if ((double)d * (1.0 - (double)T / 4294967296.0) <= 4.0) return (uint64_t)d + 1;
+        const uint64_t one_per_lane = (uint64_t)kPrefixLanes + d + 1;
+        p = p < one_per_lane ? p : one_per_lane;
+    }
+    return p > (uint64_t)d + 1 ? p : (uint64_t)d + 1;
+}
+
 struct FillArgs {
     int32_t *G;
     int64_t *row_ptr;  // nullable
@@ -99,6 +149,10 @@ hipError_t launch_fill_batch(const BatchFillArgs &a, uint64_t total_slots, uint6
                              hipStream_t stream);
 hipError_t launch_generate(uint32_t d, const GenArgs &a, uint32_t blocks, uint32_t block_threads,
                            hipStream_t stream);
+// Nodes d+1 .. a.prefix-1 of a single graph whose fill has run; one workgroup.
+hipError_t launch_prefix(uint32_t d, const PrefixArgs &a, hipStream_t stream);
+// The prefixes of `count` graphs of a batch (a.graphs[a.pidx[i]]); one workgroup each.
+hipError_t launch_prefix_batch(const PrefixArgs &a, uint32_t count, hipStream_t stream);
 hipError_t launch_generate_batch(const GenArgs &a, uint32_t blocks, uint32_t block_threads, hipStream_t stream);
 hipError_t generate_occupancy(uint32_t d, uint32_t block_threads, int *blocks_per_cu);
 hipError_t launch_degree_atomic(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, hipStream_t stream);

@@ -164,19 +164,25 @@ ppa_graph_desc desc_of(const ppa_request &r)
 int launch(ppa_server *s, Slot &sl, const std::vector<uint64_t> &ids)
 {
     std::vector<BatchGraph> graphs;
-    uint64_t work = 0;
+    uint64_t work = 0, expect_nodes = 0;  // the persistent generator's nodes; all nodes
     for (uint64_t id : ids) {
         Request &r = s->reqs[id];
-        BatchGraph g = graph_record(desc_of(r.req));
+        BatchGraph g = graph_record(desc_of(r.req), s->ctx->tuning.prefix_nodes);
         g.G = targets_of(s, r);
         g.row_ptr = row_ptr_of(s, r);
         g.targets_offset = 0;
         graphs.push_back(g);
-        work += r.req.n - r.req.d - 1;
+        work += r.req.n - g.start;
+        expect_nodes += r.req.n - r.req.d - 1;
     }
     // One request: the single-graph generator (compile-time d, full occupancy).
     // Several: the batch generator over all of them.
     const bool single = ids.size() == 1;
+    if (single) {
+        // the persistent generator's share after the LDS-resident prefix
+        const uint64_t n = graphs[0].n, start = single_start(s->ctx, desc_of(s->reqs[ids[0]].req));
+        work = n > start ? n - start : 1;
+    }
     LaunchShape shape;
     int rc = resolve_shape(s->ctx, single ? graphs[0].d : 0, work, &shape);
     if (rc) return rc;
@@ -186,31 +192,12 @@ int launch(ppa_server *s, Slot &sl, const std::vector<uint64_t> &ids)
         // at most 1/4, so the two kinds overlap instead of queueing.
         LaunchShape full;
         if ((rc = resolve_shape(s->ctx, single ? graphs[0].d : 0, ~0ull >> 1, &full)) != PPA_OK) return rc;
-        const uint32_t cap = single && work + graphs[0].d + 1 >= s->cfg.solo_nodes ? full.blocks * 3 / 4
-                                                                                    : full.blocks / 4;
+        const uint32_t cap = single && graphs[0].n >= s->cfg.solo_nodes ? full.blocks * 3 / 4 : full.blocks / 4;
         shape.blocks = std::max<uint32_t>(1u, std::min(shape.blocks, cap));
     }
     if (single) {
         const Request &r = s->reqs[ids[0]];
-        PPA_HIP(launch_reset_scratch(sl.scratch, (uint64_t)r.req.d + 1, sl.stream));
-        FillArgs fa{graphs[0].G, graphs[0].row_ptr, r.req.n, r.req.d, r.req.model};
-        PPA_HIP(launch_fill(fa, sl.stream));
-        if (work) {
-            GenArgs ga{};
-            ga.G = graphs[0].G;
-            ga.n = r.req.n;
-            ga.T = graphs[0].T;
-            ga.key0 = (uint32_t)r.req.seed;
-            ga.key1 = (uint32_t)(r.req.seed >> 32);
-            ga.chunk = shape.chunk;
-            ga.early_chunk = kEarlyChunk;
-            ga.early_nodes = shape.early_nodes;
-            ga.watchdog_ticks = shape.watchdog_ticks;
-            ga.sleep_cap = shape.sleep_cap;
-            ga.wide_mask = shape.wide_mask;
-            ga.scratch = sl.scratch;
-            PPA_HIP(launch_generate(r.req.d, ga, shape.blocks, shape.threads, sl.stream));
-        }
+        PPA_HIP(enqueue_single(s->ctx, desc_of(r.req), graphs[0].G, graphs[0].row_ptr, shape, sl.scratch, sl.stream));
     } else {
         BatchTables t;
         build_batch_tables(graphs, shape.chunk, &t);
@@ -232,7 +219,7 @@ int launch(ppa_server *s, Slot &sl, const std::vector<uint64_t> &ids)
         r.start_ms = now;
     }
     sl.ids = ids;
-    sl.expect_nodes = work;
+    sl.expect_nodes = expect_nodes;
     sl.busy = true;
     s->running += ids.size();
     return PPA_OK;

@@ -51,10 +51,10 @@ class Tuning(ctypes.Structure):
     _fields_ = [("block_threads", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
                 ("chunk_nodes", ctypes.c_uint32), ("max_blocks", ctypes.c_uint32),
                 ("watchdog_ms", ctypes.c_uint32), ("early_nodes", ctypes.c_uint32),
-                ("sleep_cap", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("sleep_cap", ctypes.c_uint32), ("prefix_nodes", ctypes.c_uint32)]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class Stats(ctypes.Structure):
@@ -232,8 +232,9 @@ class Context:
         return self._h
 
     def set_tuning(self, block_threads=0, blocks_per_cu=0, chunk_nodes=0, max_blocks=0, watchdog_ms=0,
-                   early_nodes=0, sleep_cap=0):
-        t = Tuning(block_threads, blocks_per_cu, chunk_nodes, max_blocks, watchdog_ms, early_nodes, sleep_cap)
+                   early_nodes=0, sleep_cap=0, prefix_nodes=0):
+        t = Tuning(block_threads, blocks_per_cu, chunk_nodes, max_blocks, watchdog_ms, early_nodes, sleep_cap,
+                   prefix_nodes)
         _check(lib().ppa_ctx_set_tuning(self._h, ctypes.byref(t)), "ppa_ctx_set_tuning")
 
     def get_tuning(self) -> dict:

@@ -100,7 +100,11 @@ typedef struct ppa_tuning {
     uint32_t sleep_cap;      /* longest back-off of a wavefront whose lanes all wait, in units of
                                 256 clocks, 1..64 (0 = by grid size: 1 for small grids, 32 for a
                                 full GPU) */
-    uint32_t reserved;
+    uint32_t prefix_nodes;   /* LDS-resident prefix: a single graph's first nodes generated by one
+                                workgroup with the graph in LDS, where a dependent step costs an
+                                LDS read, not a memory round trip. 0 = default (1024 + d + 1
+                                nodes when d(1-p) > 4, else none), 1 = none, N = at most N
+                                nodes (never more than 16320 / d) */
 } ppa_tuning;
 
 /* Counters of the last generate call (cuPPA_1.cu:84-87 timing CSV and the

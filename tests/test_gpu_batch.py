@@ -70,7 +70,9 @@ def test_batch_graph_replays(ctx, oracle_mod):
 
 
 @pytest.mark.parametrize("tuning", [dict(chunk_nodes=64, block_threads=64), dict(chunk_nodes=1024),
-                                    dict(max_blocks=1, block_threads=128), dict(blocks_per_cu=1, early_nodes=1)])
+                                    dict(max_blocks=1, block_threads=128), dict(blocks_per_cu=1, early_nodes=1),
+                                    dict(prefix_nodes=1),        # no LDS-resident prefixes
+                                    dict(prefix_nodes=16320)])   # every graph's prefix as large as LDS holds
 def test_batch_launch_shape_independent(ctx, oracle_mod, tuning):
     import cuppa
     specs = _specs(oracle_mod, 20, seed=5, nmax=30_000)

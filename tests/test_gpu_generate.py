@@ -54,6 +54,8 @@ def test_n1e6_d4_bit_exact(ctx, oracle_mod):
     dict(early_nodes=1),             # every row through the LDS-staged publication
     dict(early_nodes=1 << 30),       # every slot published as accepted
     dict(early_nodes=1024, chunk_nodes=64, max_blocks=2),
+    dict(prefix_nodes=1),            # no LDS-resident prefix: the persistent kernel from node d+1
+    dict(prefix_nodes=300),          # a prefix shorter than the default
 ])
 def test_launch_shape_independent(ctx, oracle_mod, tuning):
     n, d, p, seed = 200_000, 8, 0.2, 99
@@ -103,3 +105,30 @@ def test_async_launch_owns_the_context(ctx, oracle_mod):
     assert np.array_equal(t1.cpu().numpy(), want)
     assert L.ppa_generate(ctx.handle, ctypes.byref(desc), ctypes.c_void_p(t2.data_ptr()), None, s) == 0
     assert np.array_equal(t2.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("n,d,p,seed", [
+    (1020, 16, 0.1, 1),    # exactly the LDS-resident prefix (16320 / 16 nodes)
+    (1021, 16, 0.1, 2),    # one node for the persistent kernel
+    (2040, 8, 0.5, 3),
+    (16320, 1, 0.3, 4),    # d = 1: 16320 nodes in LDS, 16 per lane
+    (16321, 1, 0.3, 5),
+    (3000, 5, 0.0, 6),     # copies only
+    (5000, 3, 1.0, 7),     # direct only
+    (100_000, 16, 0.1, 8),
+])
+def test_lds_prefix_boundaries(ctx, oracle_mod, n, d, p, seed):
+    """The prefix kernel's nodes and the persistent kernel's continue the same
+    graph: with the prefix as large as LDS allows (16320 / d nodes, up to 16
+    per lane), the default, and none."""
+    T = oracle_mod.p_threshold(p)
+    want, _, want_retries = oracle_mod.generate(n, d, T, seed)
+    for off in (16320, 0, 1):
+        ctx.set_tuning(watchdog_ms=20000, prefix_nodes=off)
+        try:
+            got, rp = _gpu(ctx, n, d, T, seed)
+            st = ctx.stats()
+        finally:
+            ctx.set_tuning(watchdog_ms=20000)
+        assert np.array_equal(got, want), off
+        assert st["nodes"] == n - d - 1 and st["retries"] == want_retries
