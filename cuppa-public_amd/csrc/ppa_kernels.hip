@@ -820,26 +820,30 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
             }
         }
         if (__syncthreads_or(active) == 0) break;
-        if ((pass & 63u) == 0 && __syncthreads_or(wall_clock64() - start_clock > a.watchdog_ticks) != 0) {
-            fail = kErrTimeout;
-            break;
+        // every 64 passes, a workgroup-wide vote: a lane past the retry cap
+        // (whose slot others may be waiting for) or the wall-clock budget
+        // ends the prefix
+        if ((pass & 63u) == 0) {
+            const bool expired = wall_clock64() - start_clock > a.watchdog_ticks;
+            if (__syncthreads_or(expired || fail != 0) != 0) {
+                if (fail == 0 && expired) fail = kErrTimeout;
+                break;
+            }
         }
     }
-    __syncthreads();
+    const bool failed = __syncthreads_or(fail != 0) != 0;
     for (uint32_t s = seed_slots + threadIdx.x; s < used; s += kPrefixThreads) a.G[s] = Gs[s];
-    if (BATCH && a.done_flag && P == a.graphs[gi].n) {
+    if (fail != 0) atomicCAS(&a.scratch->status, 0, fail);
+    if (BATCH && a.done_flag && P == a.graphs[gi].n && !failed) {
         // the prefix is the whole graph: no generator pass will raise its flag
         __syncthreads();
-        if (threadIdx.x == 0 && fail == 0 && __hip_atomic_load(&a.scratch->status, __ATOMIC_RELAXED,
-                                                                __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        if (threadIdx.x == 0) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
             __hip_atomic_store(&a.done_flag[gi], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
 
     const uint32_t wnodes = wave_sum(nodes), wdraws = wave_sum(draws), wretries = wave_sum(retries);
-    const int wfail = __ballot(fail != 0) != 0 ? 1 : 0;
-    if (fail != 0) atomicCAS(&a.scratch->status, 0, fail);
     if (lane_id() == 0) {
         unsigned long long *st = a.scratch->stats;
         if (wnodes) {
@@ -849,7 +853,6 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
         if (wdraws) atomicAdd(&st[kStatDraws], (unsigned long long)wdraws);
         if (wretries) atomicAdd(&st[kStatRetries], (unsigned long long)wretries);
     }
-    (void)wfail;
 }
 
 // ---- degree pass ----------------------------------------------------------

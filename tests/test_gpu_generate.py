@@ -132,3 +132,22 @@ def test_lds_prefix_boundaries(ctx, oracle_mod, n, d, p, seed):
             ctx.set_tuning(watchdog_ms=20000)
         assert np.array_equal(got, want), off
         assert st["nodes"] == n - d - 1 and st["retries"] == want_retries
+
+
+def test_watchdog_turns_a_long_run_into_a_timeout(ctx, oracle_mod):
+    """A 1 ms budget on a graph that takes ~10 ms: the kernels see the
+    expired budget, every wavefront exits, the call returns PPA_ERR_TIMEOUT,
+    and the context generates correctly afterwards."""
+    import cuppa
+    n, d, p, seed = 20_000_000, 16, 0.1, 3
+    ctx.set_tuning(watchdog_ms=1)
+    try:
+        with pytest.raises(cuppa.PPAError) as e:
+            ctx.generate(n, d, p, seed=seed)
+        assert e.value.code == -5  # PPA_ERR_TIMEOUT
+    finally:
+        ctx.set_tuning(watchdog_ms=20000)
+    T = oracle_mod.p_threshold(p)
+    want, _, _ = oracle_mod.generate(50_000, d, T, seed)
+    got, _ = _gpu(ctx, 50_000, d, T, seed)
+    assert np.array_equal(got, want)
