@@ -304,7 +304,9 @@ def run_serve(args):
     def replay(plain, reqs, rate, arrivals_seed=1):
         rng = np.random.Generator(np.random.Philox(key=arrivals_seed))
         arr = np.cumsum(rng.exponential(1e3 / rate, len(reqs))) if rate else np.zeros(len(reqs))
-        srv = cuppa.Server(ctx, budget_bytes=budget, streams=1 if plain else args.serve_streams, plain=plain)
+        srv = cuppa.Server(ctx, budget_bytes=budget, streams=1 if plain else args.serve_streams, plain=plain,
+                           solo_nodes=0 if plain else args.serve_solo_nodes,
+                           batch_nodes=0 if plain else args.serve_batch_nodes)
         lat, makespan = srv.replay(reqs, arr.tolist())
         srv.close()
         lat = np.array(lat)
@@ -345,6 +347,7 @@ def run_serve(args):
         "config": {"workload": "requests: n lognormal(median 1e5, sigma 1.5) clipped [1e2,1e8], d {2,4,8,16}, "
                                "p {0.1,0.5,0.9}, seed = id", "requests": len(trace),
                    "budget_gb": args.serve_budget_gb, "streams": args.serve_streams,
+                   "solo_nodes": args.serve_solo_nodes, "batch_nodes": args.serve_batch_nodes,
                    "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "plain_saturation": sat, "plain": plain, "served": served,
         "throughput_ratio_at_equal_p99": (best["edges_per_s"] / plain["edges_per_s"]) if best else None,
@@ -381,6 +384,10 @@ def main():
     ap.add_argument("--serve-mults", default="1,2,3,4,5,6,7,8,9,10,12,14,16,20,24,32",
                     help="served arrival rates, as multiples of the plain rate, tried in order until the "
                          "served p99 exceeds the plain p99")
+    ap.add_argument("--serve-solo-nodes", type=int, default=0,
+                    help="requests of at least this many nodes launch alone (0 = the server's default)")
+    ap.add_argument("--serve-batch-nodes", type=int, default=0,
+                    help="a batch launch closes at this many nodes (0 = the server's default)")
     ap.add_argument("--serve-streams", type=int, default=16,
                     help="concurrent launches of the served loop (HIP hardware queues are raised to match)")
     args = ap.parse_args()

@@ -736,10 +736,13 @@ void generate_kernel(GenArgs a)
 // Same draws, same order of acceptance, hence the same graph.
 //
 // Each of the 1024 lanes owns nodes d+1+lane, +1024, ... in ascending order.
-// A pass lets every lane advance until a copy source is still empty (-1);
-// passes are separated by workgroup barriers, which also make the slots
-// written in a pass visible to the next. The smallest unfinished node only
-// depends on finished slots, so every pass finishes at least one slot.
+// On taking a node a lane draws attempt 0 of all its slots at once (the d
+// Philox calls are independent). A pass re-reads the still-empty sources of
+// all its remaining slots (LDS reads), then accepts slots in order until one
+// is still empty (-1); only a rejected duplicate costs a new draw. Passes are
+// separated by workgroup barriers, which also make the slots written in a
+// pass visible to the next. The smallest unfinished node only depends on
+// finished slots, so every pass finishes at least one slot.
 constexpr int kPrefixThreads = (int)kPrefixLanes;
 
 template <int DMAX, bool BATCH>
@@ -767,49 +770,71 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
     __syncthreads();
 
     uint64_t t = (uint64_t)d + 1 + threadIdx.x;
-    bool active = t < P, waiting = false;
-    uint32_t k = 0, r = 0, src = 0;
-    int32_t val[DMAX];
+    bool active = t < P;
+    uint32_t k = 0, r = 0;
+    // Slot i's current attempt: code[i] = ~node (direct) or its source pair
+    // (< P*d, so >= 0); cv[i] = its candidate value once known (-1 while the
+    // source is still empty). Slots below k hold accepted values in cv.
+    int32_t code[DMAX], cv[DMAX];
+    auto lds_slot = [&](int32_t c) -> int32_t {
+        return c < 0 ? ~c : __hip_atomic_load(&Gs[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    auto draw_node = [&]() {
 #pragma unroll
-    for (int i = 0; i < DMAX; ++i) val[i] = kEmpty;
+        for (int i = 0; i < DMAX; ++i) {
+            code[i] = 0;
+            cv[i] = kEmpty;
+            if ((uint32_t)i < d) {
+                const Attempt at = attempt(a.key0, a.key1, a.T, t, (uint32_t)i, 0u, t * d);
+                code[i] = at.direct ? ~(int32_t)(at.pair / d) : (int32_t)at.pair;
+                cv[i] = lds_slot(code[i]);
+            }
+        }
+    };
     uint32_t nodes = 0, draws = 0, retries = 0;
+    if (active) {
+        draw_node();
+        draws += d;
+    }
     int fail = 0;
     const uint64_t start_clock = wall_clock64();
     for (uint32_t pass = 1;; ++pass) {
+        // the empty sources of all slots from k on are re-read together, so a
+        // node waits for its slowest source, not for the sum of their waits
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < DMAX; ++i)
+                if ((uint32_t)i >= k && (uint32_t)i < d && cv[i] < 0) cv[i] = lds_slot(code[i]);
+        }
+        // accept in slot order: duplicate check against the accepted slots
         while (active) {
-            int32_t cand;
-            if (waiting) {
-                cand = __hip_atomic_load(&Gs[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (cand < 0) break;  // still empty: next pass
-                waiting = false;
-            } else {
-                const Attempt at = attempt(a.key0, a.key1, a.T, t, k, r, t * d);
-                ++draws;
-                if (at.direct) {
-                    cand = (int32_t)(at.pair / d);
-                } else {
-                    src = (uint32_t)at.pair;
-                    cand = __hip_atomic_load(&Gs[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (cand < 0) {
-                        waiting = true;
-                        break;
-                    }
-                }
-            }
+            int32_t cand = kEmpty;
+#pragma unroll
+            for (int i = 0; i < DMAX; ++i)
+                if ((uint32_t)i == k) cand = cv[i];
+            if (cand < 0) break;  // still empty: next pass
             bool dup = false;
 #pragma unroll
-            for (int i = 0; i < DMAX; ++i) dup |= (uint32_t)i < k && val[i] == cand;
+            for (int i = 0; i < DMAX; ++i) dup |= (uint32_t)i < k && cv[i] == cand;
             if (dup) {
                 ++retries;
                 if (++r >= kMaxRetry) {
                     fail = kErrRetry;
                     active = false;
+                    break;
                 }
+                const Attempt at = attempt(a.key0, a.key1, a.T, t, k, r, t * d);
+                ++draws;
+                const int32_t c = at.direct ? ~(int32_t)(at.pair / d) : (int32_t)at.pair;
+                const int32_t v = lds_slot(c);
+#pragma unroll
+                for (int i = 0; i < DMAX; ++i)
+                    if ((uint32_t)i == k) {
+                        code[i] = c;
+                        cv[i] = v;
+                    }
                 continue;
             }
-#pragma unroll
-            for (int i = 0; i < DMAX; ++i)
-                if ((uint32_t)i == k) val[i] = cand;
             __hip_atomic_store(&Gs[t * d + k], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             r = 0;
             if (++k == d) {
@@ -817,6 +842,10 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
                 k = 0;
                 t += kPrefixThreads;
                 active = t < P;
+                if (active) {
+                    draw_node();
+                    draws += d;
+                }
             }
         }
         if (__syncthreads_or(active) == 0) break;

@@ -248,7 +248,7 @@ typedef struct ppa_server_config {
                               HIP runtime has hardware queues (GPU_MAX_HW_QUEUES, default 4) */
     uint32_t max_batch;    /* requests per launch (0 = 4096) */
     uint64_t batch_nodes;  /* a launch closes once it holds this many nodes (0 = 2^22) */
-    uint64_t solo_nodes;   /* requests of at least this many nodes launch alone (0 = 2^20) */
+    uint64_t solo_nodes;   /* requests of at least this many nodes launch alone (0 = 10000) */
     uint32_t plain;        /* 1: one request per launch, one stream, single-graph kernel */
     uint32_t reserved;
 } ppa_server_config;
