@@ -236,10 +236,9 @@ def run_batch(args):
     ctx = cuppa.Context(0)
     targets = torch.empty(tot, dtype=torch.int32, device="cuda")
     row_ptr = torch.empty(rot, dtype=torch.int64, device="cuda")
-    # autotune on a sample of the ensemble (every 16th graph), then keep the shape
-    sample = specs[::16]
-    sd, st_, sr_ = cuppa.batch_layout(sample)
-    tuned = cuppa.autotune(ctx, sd, targets[:st_], row_ptr[:sr_])
+    # autotune on the ensemble itself (~20 timed launches), then keep the shape:
+    # a sample's best shape is not the whole ensemble's
+    tuned = cuppa.autotune(ctx, descs, targets, row_ptr)
     b = cuppa.Batch(ctx, descs, targets, row_ptr)
     b.run()  # warm-up
     times = []
