@@ -735,20 +735,77 @@ void generate_kernel(GenArgs a)
 // the rows out; the persistent generator starts its claims at `prefix`.
 // Same draws, same order of acceptance, hence the same graph.
 //
-// Each of the 1024 lanes owns nodes d+1+lane, +1024, ... in ascending order.
-// On taking a node a lane draws attempt 0 of all its slots at once (the d
-// Philox calls are independent). A pass re-reads the still-empty sources of
-// all its remaining slots (LDS reads), then accepts slots in order until one
-// is still empty (-1); only a rejected duplicate costs a new draw. Passes are
-// separated by workgroup barriers, which also make the slots written in a
-// pass visible to the next. The smallest unfinished node only depends on
-// finished slots, so every pass finishes at least one slot.
+// The workgroup's lanes form groups of GS >= d lanes (GS a power of two, so a
+// group lies inside one 16-lane DPP row); a group owns nodes d+1+g,
+// d+1+g+groups, ... in ascending order, one slot per lane. Attempt 0 of every
+// slot, and attempt 1 for a first duplicate, are drawn before the first pass,
+// in parallel, and parked in the empty slot; on taking a node each slot lane
+// picks them up there. A round of the group compares every
+// slot's candidate with those of the slots before it (DPP row shifts) and
+// finds the first slot from `acc` (the slots accepted so far) on that is
+// still empty or a duplicate: the slots before it are accepted at once; a
+// duplicate there, and every later slot that repeats an accepted one, draws
+// its next attempt and the group goes on; an empty source ends the group's
+// pass. A slot's work is thus spread over the node's lanes
+// instead of a loop over the node's slots on one lane. Passes re-read the
+// still-empty sources and are separated by workgroup barriers, which also
+// make the slots written in a pass visible to the next. The smallest
+// unfinished node only depends on finished slots, so every pass finishes at
+// least one slot.
 constexpr int kPrefixThreads = (int)kPrefixLanes;
+constexpr int32_t kNoCandidate = -3;  // DPP fill for lanes outside the row: equals no candidate
+// A parked draw in 15 bits: bit 14 marks a direct link to the node in bits
+// 0..13, else bits 0..13 are the source pair (both below kPrefixSlots).
+constexpr uint32_t kParkedDirect = 1u << 14;
+static_assert(kPrefixSlots <= kParkedDirect, "parked draws must fit 14 bits");
 
-template <int DMAX, bool BATCH>
+__device__ __forceinline__ uint32_t park(const Attempt &at, uint32_t d)
+{
+    // pair < P*d <= kPrefixSlots: a 32-bit quotient
+    return at.direct ? kParkedDirect | ((uint32_t)at.pair / d) : (uint32_t)at.pair;
+}
+
+// The kernel's register form: ~node (direct, < 0) or the source pair (>= 0).
+__device__ __forceinline__ int32_t unpark(uint32_t c)
+{
+    return c & kParkedDirect ? ~(int32_t)(c & (kParkedDirect - 1u)) : (int32_t)c;
+}
+
+// The smallest lane of this lane's group, below lane j, that holds v (j if
+// none). row_shr:s hands lane j the value of lane j-s of its 16-lane row
+// (kNoCandidate past the row's start); the GS-1 shifts are independent and
+// their results meet in a tree of minima, keeping the dependent chain short.
+template <int S>
+__device__ __forceinline__ uint32_t same_at(int32_t v, uint32_t j)
+{
+    const int32_t o = __builtin_amdgcn_update_dpp(kNoCandidate, v, 0x110 + S, 0xf, 0xf, false);
+    return j >= (uint32_t)S && o == v ? j - (uint32_t)S : j;
+}
+
+template <int... S>
+__device__ __forceinline__ uint32_t first_same_of(int32_t v, uint32_t j, std::integer_sequence<int, S...>)
+{
+    constexpr int N = (int)sizeof...(S) + 1;
+    uint32_t m[N] = {j, same_at<S + 1>(v, j)...};
+#pragma unroll
+    for (int w = 1; w < N; w *= 2)
+#pragma unroll
+        for (int i = 0; i + w < N; i += 2 * w) m[i] = m[i] < m[i + w] ? m[i] : m[i + w];
+    return m[0];
+}
+
+template <int GS>
+__device__ __forceinline__ uint32_t first_same(int32_t v, uint32_t j)
+{
+    return first_same_of(v, j, std::make_integer_sequence<int, GS - 1>{});
+}
+
+template <int GS, bool BATCH>
 __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
 {
+    static_assert(GS >= 1 && GS <= 16 && (GS & (GS - 1)) == 0, "a group lies inside one DPP row");
     __shared__ int32_t Gs[kPrefixSlots];
+    __shared__ int32_t s_abort;  // 0, or the error that ends the prefix
     // a batch's graph, or the single graph of the arguments
     uint32_t gi = 0;
     if (BATCH) {
@@ -759,110 +816,148 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
         a.key0 = gr.key0;
         a.key1 = gr.key1;
         a.model = gr.model;
+        a.d = gr.d;
         a.prefix = gr.start;
     }
-    const uint32_t d = BATCH ? a.graphs[gi].d : (uint32_t)DMAX;
+    const uint32_t d = a.d;
     const uint64_t P = a.prefix;
     const uint32_t seed_slots = (d + 1) * d;
     const uint32_t used = (uint32_t)P * d;
-    for (uint32_t s = threadIdx.x; s < used; s += kPrefixThreads)
-        Gs[s] = s < seed_slots ? seed_target(a.model, d, s / d, s % d) : kEmpty;
+    // Attempts 0 and 1 of every generated slot are drawn up front, by all
+    // lanes at once, and parked in the slot itself as bit 31 | attempt 1 <<
+    // 15 | attempt 0: negative, so readers still find the slot empty.
+    uint32_t draws = 0;
+    for (uint32_t s = threadIdx.x; s < used; s += kPrefixThreads) {
+        int32_t v;
+        if (s < seed_slots) {
+            v = seed_target(a.model, d, s / d, s % d);
+        } else {
+            const uint32_t tn = s / d, k = s - tn * d;
+            const Attempt a0 = attempt(a.key0, a.key1, a.T, tn, k, 0u, (uint64_t)tn * d);
+            const Attempt a1 = attempt(a.key0, a.key1, a.T, tn, k, 1u, (uint64_t)tn * d);
+            v = (int32_t)(0x80000000u | park(a1, d) << 15 | park(a0, d));
+            ++draws;
+        }
+        Gs[s] = v;
+    }
+    if (threadIdx.x == 0) s_abort = 0;
     __syncthreads();
 
-    uint64_t t = (uint64_t)d + 1 + threadIdx.x;
+    constexpr uint32_t kGroups = (uint32_t)kPrefixThreads / GS;
+    constexpr uint32_t kGroupMask = GS == 32 ? 0xffffffffu : (1u << GS) - 1u;
+    const uint32_t j = threadIdx.x % GS;                 // this lane's slot
+    const uint32_t gshift = (threadIdx.x & 63u) - j;     // the group's first lane in the wavefront
+    const bool slot_lane = j < d;
+    // Group state, the same on all its lanes: node t, its accepted slots acc.
+    uint64_t t = (uint64_t)d + 1 + threadIdx.x / GS;
     bool active = t < P;
-    uint32_t k = 0, r = 0;
-    // Slot i's current attempt: code[i] = ~node (direct) or its source pair
-    // (< P*d, so >= 0); cv[i] = its candidate value once known (-1 while the
-    // source is still empty). Slots below k hold accepted values in cv.
-    int32_t code[DMAX], cv[DMAX];
+    uint32_t acc = 0;
+    // This slot's attempt r: code = ~node (direct) or its source pair (< P*d,
+    // so >= 0); cv = its candidate value once known (kEmpty while the source
+    // is still empty).
+    uint32_t r = 0;
+    int32_t code = 0, cv = kEmpty;
     auto lds_slot = [&](int32_t c) -> int32_t {
         return c < 0 ? ~c : __hip_atomic_load(&Gs[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     };
-    auto draw_node = [&]() {
-#pragma unroll
-        for (int i = 0; i < DMAX; ++i) {
-            code[i] = 0;
-            cv[i] = kEmpty;
-            if ((uint32_t)i < d) {
-                const Attempt at = attempt(a.key0, a.key1, a.T, t, (uint32_t)i, 0u, t * d);
-                code[i] = at.direct ? ~(int32_t)(at.pair / d) : (int32_t)at.pair;
-                cv[i] = lds_slot(code[i]);
-            }
-        }
+    uint32_t nodes = 0, retries = 0;
+    int32_t spare = 0;  // attempt 1, parked with attempt 0
+    auto take = [&]() {  // this lane's slot of node t: attempt 0, from where it is parked
+        const uint32_t e =
+            (uint32_t)__hip_atomic_load(&Gs[t * d + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        code = unpark(e & 0x7fffu);
+        spare = unpark(e >> 15 & 0x7fffu);
+        cv = lds_slot(code);
     };
-    uint32_t nodes = 0, draws = 0, retries = 0;
-    if (active) {
-        draw_node();
-        draws += d;
-    }
-    int fail = 0;
+    auto draw = [&]() {  // attempt r >= 1
+        if (r == 1) {
+            code = spare;
+        } else {
+            const Attempt at = attempt(a.key0, a.key1, a.T, t, j, r, t * d);
+            code = unpark(park(at, d));
+        }
+        cv = lds_slot(code);
+        ++draws;
+    };
+    if (active && slot_lane) take();
     const uint64_t start_clock = wall_clock64();
-    for (uint32_t pass = 1;; ++pass) {
-        // the empty sources of all slots from k on are re-read together, so a
-        // node waits for its slowest source, not for the sum of their waits
-        if (active) {
-#pragma unroll
-            for (int i = 0; i < DMAX; ++i)
-                if ((uint32_t)i >= k && (uint32_t)i < d && cv[i] < 0) cv[i] = lds_slot(code[i]);
+    for (uint32_t iter = 1;; ++iter) {
+        // re-read the empty sources; the rounds run once one of them is filled
+        bool got = false;
+        if (active && slot_lane && j >= acc && cv < 0 && r < kMaxRetry) {
+            cv = lds_slot(code);
+            got = cv >= 0;
         }
-        // accept in slot order: duplicate check against the accepted slots
-        while (active) {
-            int32_t cand = kEmpty;
-#pragma unroll
-            for (int i = 0; i < DMAX; ++i)
-                if ((uint32_t)i == k) cand = cv[i];
-            if (cand < 0) break;  // still empty: next pass
-            bool dup = false;
-#pragma unroll
-            for (int i = 0; i < DMAX; ++i) dup |= (uint32_t)i < k && cv[i] == cand;
-            if (dup) {
-                ++retries;
-                if (++r >= kMaxRetry) {
-                    fail = kErrRetry;
-                    active = false;
-                    break;
-                }
-                const Attempt at = attempt(a.key0, a.key1, a.T, t, k, r, t * d);
-                ++draws;
-                const int32_t c = at.direct ? ~(int32_t)(at.pair / d) : (int32_t)at.pair;
-                const int32_t v = lds_slot(c);
-#pragma unroll
-                for (int i = 0; i < DMAX; ++i)
-                    if ((uint32_t)i == k) {
-                        code[i] = c;
-                        cv[i] = v;
-                    }
-                continue;
-            }
-            __hip_atomic_store(&Gs[t * d + k], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            r = 0;
-            if (++k == d) {
-                ++nodes;
-                k = 0;
-                t += kPrefixThreads;
-                active = t < P;
+        if (iter == 1 || __ballot(got) != 0) {
+            // rounds, until no group of the wavefront can go on
+            for (;;) {
+                const uint32_t same = first_same<GS>(cv, j);
+                const bool open = active && slot_lane && j >= acc;
+                const uint64_t pm = __ballot(open && cv < 0);
+                const uint64_t dm = __ballot(open && cv >= 0 && same < j);
+                const uint32_t gm = (uint32_t)((pm | dm) >> gshift) & kGroupMask;
+                const uint32_t first = gm ? (uint32_t)__builtin_ctz(gm) : d;
+                if (open && j < first)
+                    __hip_atomic_store(&Gs[t * d + j], cv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                bool more = false, fresh = false, redraw = false;
                 if (active) {
-                    draw_node();
-                    draws += d;
+                    if (first >= d) {
+                        // node done: take the group's next one
+                        if (j == 0) ++nodes;
+                        t += kGroups;
+                        active = t < P;
+                        acc = 0;
+                        r = 0;
+                        fresh = active && slot_lane;
+                        more = active;
+                    } else {
+                        acc = first;
+                        // the group goes on after a duplicate, not after an empty source
+                        more = ((pm >> (gshift + first)) & 1u) == 0;
+                        // every slot from `first` on whose candidate repeats an
+                        // accepted slot is rejected whatever comes between:
+                        // all of them draw again now
+                        if (open && cv >= 0 && j >= first && same < first) {
+                            ++retries;
+                            if (++r >= kMaxRetry) {
+                                cv = kEmpty;  // stays empty: the prefix is abandoned below
+                            } else {
+                                redraw = true;
+                            }
+                        }
+                    }
                 }
+                if (fresh) take();
+                if (redraw) draw();
+                if (__ballot(more) == 0) break;
+            }
+        } else {
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (__ballot(active) == 0) break;
+        // A lane past the retry cap (whose slot others may be waiting for) or
+        // the wall-clock budget ends the prefix for every wavefront.
+        const bool capped = __ballot(r >= kMaxRetry) != 0;
+        if (lane_id() == 0) {
+            int32_t err = capped ? kErrRetry : 0;
+            if (err == 0 && (iter & 63u) == 0 && wall_clock64() - start_clock > a.watchdog_ticks) err = kErrTimeout;
+            if (err != 0) {
+                int32_t none = 0;
+                __hip_atomic_compare_exchange_strong(&s_abort, &none, err, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
-        if (__syncthreads_or(active) == 0) break;
-        // every 64 passes, a workgroup-wide vote: a lane past the retry cap
-        // (whose slot others may be waiting for) or the wall-clock budget
-        // ends the prefix
-        if ((pass & 63u) == 0) {
-            const bool expired = wall_clock64() - start_clock > a.watchdog_ticks;
-            if (__syncthreads_or(expired || fail != 0) != 0) {
-                if (fail == 0 && expired) fail = kErrTimeout;
-                break;
-            }
-        }
+        if (__hip_atomic_load(&s_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0) break;
     }
-    const bool failed = __syncthreads_or(fail != 0) != 0;
-    for (uint32_t s = seed_slots + threadIdx.x; s < used; s += kPrefixThreads) a.G[s] = Gs[s];
-    if (fail != 0) atomicCAS(&a.scratch->status, 0, fail);
+    __syncthreads();
+    const int32_t fail = s_abort;
+    const bool failed = fail != 0;
+    // (a failed prefix leaves parked draws: written out as empty slots)
+    for (uint32_t s = seed_slots + threadIdx.x; s < used; s += kPrefixThreads) {
+        const int32_t v = Gs[s];
+        a.G[s] = v < 0 ? kEmpty : v;
+    }
+    if (failed && threadIdx.x == 0) atomicCAS(&a.scratch->status, 0, fail);
     if (BATCH && a.done_flag && P == a.graphs[gi].n && !failed) {
         // the prefix is the whole graph: no generator pass will raise its flag
         __syncthreads();
@@ -969,10 +1064,11 @@ hipError_t launch_generate_d(const GenArgs &a, uint32_t blocks, uint32_t threads
     return hipGetLastError();
 }
 
-template <int D>
-hipError_t launch_prefix_d(const PrefixArgs &a, hipStream_t stream)
+template <int GS>
+hipError_t launch_prefix_gs(PrefixArgs a, uint32_t d, hipStream_t stream)
 {
-    hipLaunchKernelGGL((prefix_kernel<D, false>), dim3(1), dim3(kPrefixThreads), 0, stream, a);
+    a.d = d;
+    hipLaunchKernelGGL((prefix_kernel<GS, false>), dim3(1), dim3(kPrefixThreads), 0, stream, a);
     return hipGetLastError();
 }
 
@@ -1023,18 +1119,19 @@ hipError_t launch_generate(uint32_t d, const GenArgs &a, uint32_t blocks, uint32
 
 hipError_t launch_prefix(uint32_t d, const PrefixArgs &a, hipStream_t stream)
 {
-    switch (d) {
-#define PPA_CASE(DD) \
-    case DD: return launch_prefix_d<DD>(a, stream);
-        PPA_FOR_EACH_D(PPA_CASE)
-#undef PPA_CASE
-    default: return hipErrorInvalidValue;
-    }
+    // groups of the next power of two >= d lanes
+    if (d == 0 || d > 16) return hipErrorInvalidValue;
+    if (d == 1) return launch_prefix_gs<1>(a, d, stream);
+    if (d == 2) return launch_prefix_gs<2>(a, d, stream);
+    if (d <= 4) return launch_prefix_gs<4>(a, d, stream);
+    if (d <= 8) return launch_prefix_gs<8>(a, d, stream);
+    return launch_prefix_gs<16>(a, d, stream);
 }
 
 hipError_t launch_prefix_batch(const PrefixArgs &a, uint32_t count, hipStream_t stream)
 {
-    hipLaunchKernelGGL((prefix_kernel<kBatchDMax, true>), dim3(count), dim3(kPrefixThreads), 0, stream, a);
+    static_assert(kBatchDMax == 16, "batch prefix groups are 16 lanes");
+    hipLaunchKernelGGL((prefix_kernel<16, true>), dim3(count), dim3(kPrefixThreads), 0, stream, a);
     return hipGetLastError();
 }
 

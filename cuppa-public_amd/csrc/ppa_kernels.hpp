@@ -87,7 +87,7 @@ struct PrefixArgs {
     uint64_t T;
     uint32_t key0, key1;
     uint32_t model;           // PPA_MODEL_*: which seed rows
-    uint32_t pad;
+    uint32_t d;               // set by the launchers (batch: per graph)
     uint64_t prefix;          // nodes [d+1, prefix) are generated; prefix * d <= kPrefixSlots
     uint64_t watchdog_ticks;
     DeviceScratch *scratch;
@@ -96,9 +96,9 @@ struct PrefixArgs {
     uint32_t *done_flag;      // batch: per-graph host flag, raised here for a graph the prefix completes
 };
 
-// int32 slots of the LDS-resident prefix: 64 KB of LDS less the workgroup
-// reduction's 256 bytes. It holds prefix_nodes(n, d) nodes, over the
-// kPrefixLanes lanes of its one workgroup.
+// int32 slots of the LDS-resident prefix: with its 4-byte abort word, 65,284
+// of the workgroup's 65,536 bytes of LDS. It holds prefix_nodes(n, d) nodes,
+// over the kPrefixLanes lanes of its one workgroup.
 constexpr uint32_t kPrefixSlots = 16320;
 constexpr uint32_t kPrefixLanes = 1024;
 inline uint64_t prefix_nodes(uint64_t n, uint32_t d)
@@ -110,9 +110,9 @@ inline uint64_t prefix_nodes(uint64_t n, uint32_t d)
 // LDS-resident prefix, or d + 1 without one. prefix_cap is the tuning's
 // prefix_nodes: 0 = default (1024 + d + 1 nodes where a node has more than 4
 // copy draws on average, d(1-p) > 4, else none), 1 = none, N = at most N.
-// Measured at n = 1e5 .. 1e7: above 4 copies per node the prefix takes 6-22 %
-// off a call (d = 16, p = 0.1, n = 1e5: 5.45 -> 4.2 ms); at 4 or below it
-// costs up to 8 %, and more nodes per lane never help.
+// Measured at n = 2e4 .. 1e7: above 4 copies per node the prefix takes 20-60 %
+// off a call (d = 16, p = 0.1, n = 1e5: 4.0 -> 1.9 ms); at 4 or below it
+// costs 5-15 %, and a longer prefix never helps.
 inline uint64_t prefix_start(uint64_t n, uint32_t d, uint64_t T, uint32_t prefix_cap)
 {
     if (prefix_cap == 1) return (uint64_t)d + 1;
