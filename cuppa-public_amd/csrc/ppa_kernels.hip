@@ -153,10 +153,23 @@ __device__ __forceinline__ uint64_t find_segment(const uint64_t *base, uint64_t 
     return lo;
 }
 
-constexpr uint32_t kFillPerThread = 8;
+constexpr uint32_t kFillPerThread = 64;  // batch fill: a span of 64 KB per block and step
 
 // The fill of every graph of a batch over one flat index space of slots and
-// one of row offsets; a block locates its first graph once.
+// one of row offsets. A block locates the graph of its span once (a binary
+// search over the graphs, whose latency the 64 KB span amortises: 8 KB
+// spans filled at 4.4 TB/s, 64 KB ones at 6.4); a span inside one graph
+// (nearly all of them: graphs hold up to 1.6e8 slots) is filled without
+// per-slot lookups, and only the (d+1)*d seed slots at a graph's start
+// compute a value.
+__device__ __forceinline__ int32_t fill_value(const BatchGraph &gr, uint64_t local)
+{
+    const uint32_t clique = (gr.d + 1) * gr.d;
+    if (local >= clique) return kEmpty;
+    const uint32_t l = (uint32_t)local;
+    return seed_target(gr.model, gr.d, l / gr.d, l % gr.d);
+}
+
 __global__ __launch_bounds__(256) void fill_batch_kernel(BatchFillArgs a, uint64_t total_slots,
                                                           uint64_t total_rows)
 {
@@ -166,28 +179,51 @@ __global__ __launch_bounds__(256) void fill_batch_kernel(BatchFillArgs a, uint64
              g += (uint64_t)gridDim.x * blockDim.x)
             a.done_nodes[g] = 0;
     for (uint64_t base = blockIdx.x * per_block; base < total_slots; base += gridDim.x * per_block) {
+        const uint64_t end = base + per_block < total_slots ? base + per_block : total_slots;
         uint64_t g = find_segment(a.slot_base, a.count, base);
-        for (uint32_t j = 0; j < kFillPerThread; ++j) {
-            const uint64_t s = base + (uint64_t)j * blockDim.x + threadIdx.x;
-            if (s >= total_slots) break;
-            while (s >= a.slot_base[g + 1]) ++g;
+        if (a.slot_base[g + 1] >= end) {
             const BatchGraph &gr = a.graphs[g];
-            const uint64_t local = s - a.slot_base[g];
-            const uint64_t clique = (uint64_t)(gr.d + 1) * gr.d;
-            gr.G[local] =
-                local < clique ? seed_target(gr.model, gr.d, local / gr.d, (uint32_t)(local % gr.d)) : kEmpty;
+            int32_t *const G = gr.G;
+            const uint64_t first = base - a.slot_base[g];
+            for (uint32_t j = 0; j < kFillPerThread; ++j) {
+                const uint64_t i = (uint64_t)j * blockDim.x + threadIdx.x;
+                if (base + i >= end) break;
+                G[first + i] = fill_value(gr, first + i);
+            }
+        } else {
+            for (uint32_t j = 0; j < kFillPerThread; ++j) {
+                const uint64_t s = base + (uint64_t)j * blockDim.x + threadIdx.x;
+                if (s >= end) break;
+                while (s >= a.slot_base[g + 1]) ++g;
+                const BatchGraph &gr = a.graphs[g];
+                const uint64_t local = s - a.slot_base[g];
+                gr.G[local] = fill_value(gr, local);
+            }
         }
     }
     for (uint64_t base = blockIdx.x * per_block; base < total_rows; base += gridDim.x * per_block) {
+        const uint64_t end = base + per_block < total_rows ? base + per_block : total_rows;
         uint64_t g = find_segment(a.row_base, a.count, base);
-        for (uint32_t j = 0; j < kFillPerThread; ++j) {
-            const uint64_t s = base + (uint64_t)j * blockDim.x + threadIdx.x;
-            if (s >= total_rows) break;
-            while (s >= a.row_base[g + 1]) ++g;
+        if (a.row_base[g + 1] >= end) {
             const BatchGraph &gr = a.graphs[g];
             if (gr.row_ptr) {
-                const uint64_t t = s - a.row_base[g];
-                gr.row_ptr[t] = (int64_t)(gr.targets_offset + t * gr.d);
+                const uint64_t first = base - a.row_base[g];
+                for (uint32_t j = 0; j < kFillPerThread; ++j) {
+                    const uint64_t i = (uint64_t)j * blockDim.x + threadIdx.x;
+                    if (base + i >= end) break;
+                    gr.row_ptr[first + i] = (int64_t)(gr.targets_offset + (first + i) * gr.d);
+                }
+            }
+        } else {
+            for (uint32_t j = 0; j < kFillPerThread; ++j) {
+                const uint64_t s = base + (uint64_t)j * blockDim.x + threadIdx.x;
+                if (s >= end) break;
+                while (s >= a.row_base[g + 1]) ++g;
+                const BatchGraph &gr = a.graphs[g];
+                if (gr.row_ptr) {
+                    const uint64_t t = s - a.row_base[g];
+                    gr.row_ptr[t] = (int64_t)(gr.targets_offset + t * gr.d);
+                }
             }
         }
     }
