@@ -305,7 +305,8 @@ def run_serve(args):
         arr = np.cumsum(rng.exponential(1e3 / rate, len(reqs))) if rate else np.zeros(len(reqs))
         srv = cuppa.Server(ctx, budget_bytes=budget, streams=1 if plain else args.serve_streams, plain=plain,
                            solo_nodes=0 if plain else args.serve_solo_nodes,
-                           batch_nodes=0 if plain else args.serve_batch_nodes)
+                           batch_nodes=0 if plain else args.serve_batch_nodes,
+                           solo_grid_div=0 if plain else args.serve_solo_grid_div)
         lat, makespan = srv.replay(reqs, arr.tolist())
         srv.close()
         lat = np.array(lat)
@@ -347,6 +348,7 @@ def run_serve(args):
                                "p {0.1,0.5,0.9}, seed = id", "requests": len(trace),
                    "budget_gb": args.serve_budget_gb, "streams": args.serve_streams,
                    "solo_nodes": args.serve_solo_nodes, "batch_nodes": args.serve_batch_nodes,
+                   "solo_grid_div": args.serve_solo_grid_div,
                    "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "plain_saturation": sat, "plain": plain, "served": served,
         "throughput_ratio_at_equal_p99": (best["edges_per_s"] / plain["edges_per_s"]) if best else None,
@@ -387,6 +389,8 @@ def main():
                     help="requests of at least this many nodes launch alone (0 = the server's default)")
     ap.add_argument("--serve-batch-nodes", type=int, default=0,
                     help="a batch launch closes at this many nodes (0 = the server's default)")
+    ap.add_argument("--serve-solo-grid-div", type=int, default=0,
+                    help="lone launches get 1/this of the single-call grid (0 = the server's default)")
     ap.add_argument("--serve-streams", type=int, default=16,
                     help="concurrent launches of the served loop (HIP hardware queues are raised to match)")
     args = ap.parse_args()

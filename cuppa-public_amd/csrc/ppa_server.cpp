@@ -194,6 +194,10 @@ int launch(ppa_server *s, Slot &sl, const std::vector<uint64_t> &ids)
         if ((rc = resolve_shape(s->ctx, single ? graphs[0].d : 0, ~0ull >> 1, &full)) != PPA_OK) return rc;
         const uint32_t cap = single && graphs[0].n >= s->cfg.solo_nodes ? full.blocks * 3 / 4 : full.blocks / 4;
         shape.blocks = std::max<uint32_t>(1u, std::min(shape.blocks, cap));
+        // A lone request's grid from the single-call rule is sized for its
+        // latency on an idle GPU; here up to `streams` launches share it, and
+        // a chain-bound grid mostly waits while holding its wave slots.
+        if (single) shape.blocks = std::max<uint32_t>(1u, shape.blocks / s->cfg.solo_grid_div);
     }
     if (single) {
         const Request &r = s->reqs[ids[0]];
@@ -247,9 +251,15 @@ int ppa_server_create(ppa_ctx *ctx, const ppa_server_config *cfg, ppa_server **o
     // Requests of 1e4 nodes and more launch alone: with the single-graph
     // kernel's grid sized to the request and the LDS-resident prefix, a lone
     // launch beats a place in a batch launch from there on (served at equal
-    // p99: 7.1x the plain rate, against 3.6x when requests below 2^20 nodes
-    // were batched); batches collect the smallest requests.
+    // p99, when this was chosen: 7.1x the plain rate, against 3.6x when
+    // requests below 2^20 nodes were batched); batches collect the smallest
+    // requests.
     if (!s->cfg.solo_nodes) s->cfg.solo_nodes = 10000;
+    // A third of the single-call grid for lone launches: at 16 streams the
+    // served rate at equal p99 rose from 5.3x to 6.2x the plain rate (a half
+    // and a quarter: also 6.2x; an eighth: below 5x, the requests' own chains
+    // slow down).
+    if (!s->cfg.solo_grid_div) s->cfg.solo_grid_div = 3;
     const uint64_t budget = s->cfg.budget_bytes / kGranule * kGranule;
     if (hipMalloc(&s->pool_base, budget) != hipSuccess) {
         destroy_server(s);

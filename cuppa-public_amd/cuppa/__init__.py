@@ -73,7 +73,7 @@ class Stats(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_uint64), ("streams", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
                 ("batch_nodes", ctypes.c_uint64), ("solo_nodes", ctypes.c_uint64), ("plain", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("solo_grid_div", ctypes.c_uint32)]
 
 
 class Request(ctypes.Structure):
@@ -389,10 +389,11 @@ class Server:
     """Continuous-batching request loop (ppa_server)."""
 
     def __init__(self, ctx: "Context", budget_bytes: int, streams: int = 0, max_batch: int = 0,
-                 batch_nodes: int = 0, solo_nodes: int = 0, plain: bool = False):
+                 batch_nodes: int = 0, solo_nodes: int = 0, plain: bool = False, solo_grid_div: int = 0):
         self.ctx = ctx
         self._h = ctypes.c_void_p()
-        cfg = ServerConfig(budget_bytes, streams, max_batch, batch_nodes, solo_nodes, 1 if plain else 0, 0)
+        cfg = ServerConfig(budget_bytes, streams, max_batch, batch_nodes, solo_nodes, 1 if plain else 0,
+                           solo_grid_div)
         _check(lib().ppa_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(self._h)), "ppa_server_create")
 
     def submit(self, rid: int, n: int, d: int, p_thresh: int, seed: int, model: int = MODEL_COPY) -> int:

@@ -250,7 +250,10 @@ typedef struct ppa_server_config {
     uint64_t batch_nodes;  /* a launch closes once it holds this many nodes (0 = 2^22) */
     uint64_t solo_nodes;   /* requests of at least this many nodes launch alone (0 = 10000) */
     uint32_t plain;        /* 1: one request per launch, one stream, single-graph kernel */
-    uint32_t reserved;
+    uint32_t solo_grid_div; /* a launch of one request gets 1/solo_grid_div of the workgroups a
+                               single ppa_generate call would (0 = 3; 1 = the same): its
+                               chain-bound workgroups hold wave slots that concurrent launches
+                               could use; ignored in plain mode */
 } ppa_server_config;
 
 typedef struct ppa_request {
