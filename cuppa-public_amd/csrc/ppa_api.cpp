@@ -78,18 +78,27 @@ uint64_t single_start(const ppa_ctx *ctx, const ppa_graph_desc &desc)
     return prefix_start(desc.n, desc.d, coin_threshold(desc.model, desc.p_threshold), ctx->tuning.prefix_nodes);
 }
 
-hipError_t enqueue_single(const ppa_ctx *ctx, const ppa_graph_desc &desc, int32_t *G, int64_t *row_ptr,
-                          const LaunchShape &shape, DeviceScratch *scratch, hipStream_t stream, hipEvent_t after_fill)
+uint64_t range_start(const ppa_ctx *ctx, const ppa_graph_desc &desc, uint64_t first, uint64_t last)
 {
-    const uint64_t n = desc.n;
+    if (first > 0) return first;
+    const uint64_t p = single_start(ctx, desc);
+    const uint64_t seeds = (uint64_t)desc.d + 1;
+    return p < last ? p : (last > seeds ? last : seeds);
+}
+
+hipError_t enqueue_single(const ppa_ctx *ctx, const ppa_graph_desc &desc, int32_t *G, int64_t *row_ptr,
+                          const LaunchShape &shape, DeviceScratch *scratch, hipStream_t stream, hipEvent_t after_fill,
+                          uint64_t first, uint64_t last)
+{
     const uint32_t d = desc.d;
-    const uint64_t start = single_start(ctx, desc);
+    const uint64_t n = last ? last : desc.n;  // the generator stops at node n
+    const uint64_t start = range_start(ctx, desc, first, n);
     const uint64_t T = coin_threshold(desc.model, desc.p_threshold);
     hipError_t e = launch_reset_scratch(scratch, start, stream);
     if (e != hipSuccess) return e;
-    if ((e = launch_fill(FillArgs{G, row_ptr, n, d, desc.model}, stream)) != hipSuccess) return e;
+    if (first == 0 && (e = launch_fill(FillArgs{G, row_ptr, desc.n, d, desc.model}, stream)) != hipSuccess) return e;
     if (after_fill && (e = hipEventRecord(after_fill, stream)) != hipSuccess) return e;
-    if (start > (uint64_t)d + 1) {
+    if (first == 0 && start > (uint64_t)d + 1) {
         PrefixArgs pa{};
         pa.G = G;
         pa.T = T;
@@ -295,6 +304,42 @@ int ppa_generate_async(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targ
     PPA_HIP(hipEventRecord(ctx->ev[2], stream));
     ctx->pending = true;
     ctx->expect_nodes = n - d - 1;
+    ctx->range_next = 0;
+    return PPA_OK;
+}
+
+int ppa_generate_range(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets, int64_t *d_row_ptr,
+                       uint64_t first, uint64_t last, void *stream_)
+{
+    if (!ctx || !d_targets) return PPA_ERR_INVALID;
+    int rc = ppa_check_desc(desc);
+    if (rc) return rc;
+    if (((uintptr_t)d_targets & 15u) != 0 || last <= first || last > desc->n) return PPA_ERR_INVALID;
+    if (ctx->pending) return PPA_ERR_AGAIN;
+    // a later range continues the graph of the previous one, where it ended
+    if (first > 0 && (first != ctx->range_next || d_targets != ctx->range_targets ||
+                      std::memcmp(desc, &ctx->range_desc, sizeof(*desc)) != 0))
+        return PPA_ERR_INVALID;
+    DeviceGuard g(ctx->device);
+    if (!g.ok()) return PPA_ERR_DEVICE;
+    hipStream_t stream = (hipStream_t)stream_;
+    const uint32_t d = desc->d;
+    const uint64_t start = range_start(ctx, *desc, first, last);
+    LaunchShape shape;
+    if ((rc = resolve_shape(ctx, d, last > start ? last - start : 1, &shape)) != PPA_OK) return rc;
+
+    ctx->last = ppa_stats{};
+    ctx->last.grid_blocks = shape.blocks;
+    ctx->last.block_threads = shape.threads;
+    PPA_HIP(hipEventRecord(ctx->ev[0], stream));
+    PPA_HIP(enqueue_single(ctx, *desc, d_targets, d_row_ptr, shape, ctx->scratch, stream, ctx->ev[1], first, last));
+    PPA_HIP(hipEventRecord(ctx->ev[2], stream));
+    ctx->pending = true;
+    const uint64_t lo = first > (uint64_t)d + 1 ? first : (uint64_t)d + 1;
+    ctx->expect_nodes = last > lo ? last - lo : 0;
+    ctx->range_desc = *desc;
+    ctx->range_targets = d_targets;
+    ctx->range_next = last == desc->n ? 0 : last;
     return PPA_OK;
 }
 

@@ -25,6 +25,11 @@ struct ppa_ctx {
     ppa_stats last{};
     bool pending = false;        // a launch awaits ppa_ctx_sync / ppa_batch_sync
     uint64_t expect_nodes = 0;   // generated nodes the pending launch must finish
+    // ppa_generate_range: the graph being generated range by range, and the
+    // node its next range must start at (0: none in progress)
+    ppa_graph_desc range_desc{};
+    int32_t *range_targets = nullptr;
+    uint64_t range_next = 0;
     std::map<std::string, ppa_tuning> tune_cache;
 };
 
@@ -74,11 +79,16 @@ int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *ou
 // LDS-resident prefix, or d + 1 without it.
 uint64_t single_start(const ppa_ctx *ctx, const ppa_graph_desc &desc);
 // Reset, fill, prefix and persistent generator of one graph on `stream`;
-// `shape` was resolved for the work n - single_start(ctx, n, d). after_fill
-// (nullable) is recorded between the fill and the generation.
+// `shape` was resolved for the work range_start(...) .. last. after_fill
+// (nullable) is recorded between the fill and the generation. A node range
+// [first, last) (last = 0: n) generates only those nodes: first = 0 fills the
+// whole graph and runs the prefix (cut at `last`); first > 0 needs nodes below
+// it final and the graph filled.
 hipError_t enqueue_single(const ppa_ctx *ctx, const ppa_graph_desc &desc, int32_t *G, int64_t *row_ptr,
                           const LaunchShape &shape, DeviceScratch *scratch, hipStream_t stream,
-                          hipEvent_t after_fill = nullptr);
+                          hipEvent_t after_fill = nullptr, uint64_t first = 0, uint64_t last = 0);
+// The first node the persistent generator claims in the range [first, last).
+uint64_t range_start(const ppa_ctx *ctx, const ppa_graph_desc &desc, uint64_t first, uint64_t last);
 
 // Grows the context's work area to `need` bytes (waiting for `stream` before
 // freeing the old one); PPA_ERR_NOMEM if the device cannot provide it.

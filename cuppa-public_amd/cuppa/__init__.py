@@ -106,6 +106,8 @@ _SIGNATURES = [
     ("ppa_generate", ctypes.c_int, [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP]),
     ("ppa_generate_async", ctypes.c_int, [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP]),
     ("ppa_ctx_sync", ctypes.c_int, [_VP, _VP]),
+    ("ppa_generate_range", ctypes.c_int,
+     [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, ctypes.c_uint64, ctypes.c_uint64, _VP]),
     ("ppa_degree_histogram", ctypes.c_int,
      [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP, ctypes.c_uint64, _VP, _VP]),
     ("ppa_histogram_of_degrees", ctypes.c_int, [_VP, ctypes.c_uint64, _VP, _VP, ctypes.c_uint64, _VP, _VP]),
@@ -257,6 +259,15 @@ class Context:
 
     def sync(self, stream=None):
         _check(lib().ppa_ctx_sync(self._h, _stream_handle(stream)), "ppa_ctx_sync")
+
+    def generate_range(self, desc: GraphDesc, targets, row_ptr, first: int, last: int, stream=None):
+        """Enqueues nodes [first, last) of a graph generated range by range
+        (ppa_generate_range); sync() ends it, after which rows < last are final."""
+        tp = targets.data_ptr() if hasattr(targets, "data_ptr") else int(targets)
+        rp = (row_ptr.data_ptr() if hasattr(row_ptr, "data_ptr") else int(row_ptr)) if row_ptr is not None else None
+        _check(lib().ppa_generate_range(self._h, ctypes.byref(desc), ctypes.c_void_p(tp),
+                                        ctypes.c_void_p(rp) if rp else None, first, last, _stream_handle(stream)),
+               "ppa_generate_range")
 
     def generate(self, n: int, d: int, p: float | None = None, seed: int = 0, p_thresh: int | None = None,
                  row_ptr: bool = True, stream=None, model: int = MODEL_COPY) -> Graph:

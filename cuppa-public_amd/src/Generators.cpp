@@ -48,40 +48,59 @@ bool writeEdgeList(const std::string &path, const int32_t *G, uint64_t n, uint32
 
 namespace {
 
-// Writes `bytes` from device memory, double-buffered through pinned memory.
-bool writeDevice(FILE *f, const void *dptr, size_t bytes)
-{
-    constexpr size_t kChunk = 64u << 20;
-    void *buf[2] = {nullptr, nullptr};
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    bool ok = hipHostMalloc(&buf[0], kChunk, hipHostMallocDefault) == hipSuccess &&
-              hipHostMalloc(&buf[1], kChunk, hipHostMallocDefault) == hipSuccess &&
-              hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess &&
-              hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) == hipSuccess &&
-              hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) == hipSuccess;
-    const char *src = (const char *)dptr;
-    const size_t chunks = (bytes + kChunk - 1) / kChunk;
-    auto issue = [&](size_t c) {
-        const size_t off = c * kChunk, len = std::min(kChunk, bytes - off);
-        return hipMemcpyAsync(buf[c & 1], src + off, len, hipMemcpyDeviceToHost, stream) == hipSuccess &&
-               hipEventRecord(ev[c & 1], stream) == hipSuccess;
-    };
-    if (ok && chunks) ok = issue(0);
-    for (size_t c = 0; ok && c < chunks; ++c) {
-        ok = hipEventSynchronize(ev[c & 1]) == hipSuccess;
-        if (ok && c + 1 < chunks) ok = issue(c + 1);  // next copy runs while this chunk is written
-        const size_t len = std::min(kChunk, bytes - c * kChunk);
-        if (ok) ok = std::fwrite(buf[c & 1], 1, len, f) == len;
+// Writes device memory to a file, double-buffered through two pinned 64 MB
+// buffers: the copy of one chunk runs while the previous one is written. The
+// buffers are allocated once per writer (pinned allocations are slow).
+class DeviceWriter {
+public:
+    DeviceWriter()
+    {
+        ok_ = hipHostMalloc(&buf_[0], kChunk, hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc(&buf_[1], kChunk, hipHostMallocDefault) == hipSuccess &&
+              hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming) == hipSuccess;
     }
-    if (stream) (void)hipStreamSynchronize(stream);
-    for (int i = 0; i < 2; ++i) {
-        if (ev[i]) (void)hipEventDestroy(ev[i]);
-        if (buf[i]) (void)hipHostFree(buf[i]);
+    ~DeviceWriter()
+    {
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        for (int i = 0; i < 2; ++i) {
+            if (ev_[i]) (void)hipEventDestroy(ev_[i]);
+            if (buf_[i]) (void)hipHostFree(buf_[i]);
+        }
+        if (stream_) (void)hipStreamDestroy(stream_);
     }
-    if (stream) (void)hipStreamDestroy(stream);
-    return ok;
-}
+    DeviceWriter(const DeviceWriter &) = delete;
+    DeviceWriter &operator=(const DeviceWriter &) = delete;
+
+    bool write(FILE *f, const void *dptr, size_t bytes)
+    {
+        const char *src = (const char *)dptr;
+        const size_t chunks = (bytes + kChunk - 1) / kChunk;
+        auto issue = [&](size_t c) {
+            const size_t off = c * kChunk, len = std::min(kChunk, bytes - off);
+            return hipMemcpyAsync(buf_[c & 1], src + off, len, hipMemcpyDeviceToHost, stream_) == hipSuccess &&
+                   hipEventRecord(ev_[c & 1], stream_) == hipSuccess;
+        };
+        bool ok = ok_;
+        if (ok && chunks) ok = issue(0);
+        for (size_t c = 0; ok && c < chunks; ++c) {
+            ok = hipEventSynchronize(ev_[c & 1]) == hipSuccess;
+            if (ok && c + 1 < chunks) ok = issue(c + 1);  // next copy runs while this chunk is written
+            const size_t len = std::min(kChunk, bytes - c * kChunk);
+            if (ok) ok = std::fwrite(buf_[c & 1], 1, len, f) == len;
+        }
+        if (stream_) (void)hipStreamSynchronize(stream_);  // no copy outlives the call
+        return ok;
+    }
+
+private:
+    static constexpr size_t kChunk = 64u << 20;
+    void *buf_[2] = {nullptr, nullptr};
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    bool ok_ = false;
+};
 
 }  // namespace
 
@@ -93,7 +112,8 @@ bool writeCsr(const std::string &path, const CsrHeader &h, const int64_t *offset
     bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
     const size_t ob = (h.n + 1) * sizeof(int64_t), ib = h.nnz * sizeof(int32_t);
     if (device) {
-        ok = ok && writeDevice(f, offsets, ob) && writeDevice(f, indices, ib);
+        DeviceWriter w;
+        ok = ok && w.write(f, offsets, ob) && w.write(f, indices, ib);
     } else {
         ok = ok && std::fwrite(offsets, 1, ob, f) == ob && std::fwrite(indices, 1, ib, f) == ib;
     }
@@ -147,9 +167,10 @@ int deviceDegreeHistogram(ppa_ctx *ctx, const ppa_graph_desc &g, const int32_t *
     return rc;
 }
 
-// --edges-file, --csr-file and --in-csr-file of a graph held on the device.
+// --edges-file, --csr-file (unless csrDone) and --in-csr-file of a graph
+// held on the device.
 int writeDeviceGraphFiles(const ExperimentConfiguration &conf, ppa_ctx *ctx, const ppa_graph_desc &g,
-                          const int32_t *dGraph, const int64_t *dRowPtr)
+                          const int32_t *dGraph, const int64_t *dRowPtr, bool csrDone = false)
 {
     int rc = PPA_OK;
     if (!conf._edgesFile.empty()) {
@@ -158,7 +179,8 @@ int writeDeviceGraphFiles(const ExperimentConfiguration &conf, ppa_ctx *ctx, con
             return PPA_ERR_DEVICE;
         if (!writeEdgeList(conf._edgesFile, host.data(), g.n, g.d, g.model)) return PPA_ERR_INVALID;
     }
-    if (!conf._csrFile.empty() && !writeCsr(conf._csrFile, csrHeader(g, g.n * g.d, 0), dRowPtr, dGraph, true))
+    if (!csrDone && !conf._csrFile.empty() &&
+        !writeCsr(conf._csrFile, csrHeader(g, g.n * g.d, 0), dRowPtr, dGraph, true))
         return PPA_ERR_INVALID;
     if (!conf._inCsrFile.empty()) {
         int64_t *dPtr = nullptr;
@@ -239,9 +261,15 @@ int cuPPA::run(const ExperimentConfiguration &conf, const char *timingCsv, bool 
     }
     std::printf("--------------------------------------------------------------------------------\n");
     std::printf("Graph: %.2f MB\t| RowPtr: %.2f MB\n", g.n * g.d * 4.0 / 1048576.0, (g.n + 1) * 8.0 / 1048576.0);
-    rc = ppa_generate(_ctx, &g, _dGraph, _dRowPtr, nullptr);
+    // with --csr-file (and one launch shape) the file is written while the graph is generated
+    const bool streamCsr = !conf._csrFile.empty() && !adaptive;
     ppa_stats st{};
-    ppa_get_stats(_ctx, &st);
+    if (streamCsr) {
+        rc = generateStreamingCsr(conf, g, &st);
+    } else {
+        rc = ppa_generate(_ctx, &g, _dGraph, _dRowPtr, nullptr);
+        ppa_get_stats(_ctx, &st);
+    }
     if (rc) return rc;
     std::vector<ppa_stats> trials{st};
     if (adaptive && !conf._blocksGiven) {
@@ -288,9 +316,55 @@ int cuPPA::run(const ExperimentConfiguration &conf, const char *timingCsv, bool 
     }
     std::printf("Computation Time: %f\n\n\n", st.total_ms * 1e-3);
     if (conf._FLAG_SHOW_DEGREE_DISTRIBUTION && (rc = printDegreeDistribution(conf)) != PPA_OK) return rc;
-    if ((rc = writeDeviceGraphFiles(conf, _ctx, g, _dGraph, _dRowPtr)) != PPA_OK) return rc;
+    if ((rc = writeDeviceGraphFiles(conf, _ctx, g, _dGraph, _dRowPtr, streamCsr)) != PPA_OK) return rc;
     std::cout << "Ended" << std::endl;
     return PPA_OK;
+}
+
+int cuPPA::generateStreamingCsr(const ExperimentConfiguration &conf, const ppa_graph_desc &g, ppa_stats *total)
+{
+    FILE *f = std::fopen(conf._csrFile.c_str(), "wb");
+    if (!f) return PPA_ERR_INVALID;
+    const CsrHeader h = csrHeader(g, g.n * g.d, 0);
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+    DeviceWriter w;
+    hipStream_t gen = nullptr;
+    int rc = hipStreamCreateWithFlags(&gen, hipStreamNonBlocking) == hipSuccess ? PPA_OK : PPA_ERR_DEVICE;
+    // eight ranges, of at least 2^20 nodes
+    const uint64_t step = std::max<uint64_t>(1ull << 20, (g.n + 7) / 8);
+    std::vector<uint64_t> bounds{0};
+    while (bounds.back() < g.n) bounds.push_back(std::min<uint64_t>(g.n, bounds.back() + step));
+    *total = ppa_stats{};
+    auto finish = [&]() {
+        int r = ppa_ctx_sync(_ctx, gen);
+        ppa_stats s{};
+        ppa_get_stats(_ctx, &s);
+        total->total_ms += s.total_ms;
+        total->kernel_ms += s.kernel_ms;
+        total->draws += s.draws;
+        total->retries += s.retries;
+        total->prefetch_misses += s.prefetch_misses;
+        total->grid_blocks = std::max(total->grid_blocks, s.grid_blocks);
+        total->block_threads = s.block_threads;
+        return r;
+    };
+    if (!rc) rc = ppa_generate_range(_ctx, &g, _dGraph, _dRowPtr, 0, bounds[1], gen);
+    if (!rc) rc = finish();
+    for (size_t k = 1; !rc && ok && k < bounds.size(); ++k) {
+        const bool more = k + 1 < bounds.size();
+        if (more) rc = ppa_generate_range(_ctx, &g, _dGraph, _dRowPtr, bounds[k], bounds[k + 1], gen);
+        // while range k runs: the row offsets (filled by range 0), then range k-1's rows
+        if (!rc && k == 1) ok = w.write(f, _dRowPtr, (g.n + 1) * sizeof(int64_t));
+        if (!rc && ok)
+            ok = w.write(f, _dGraph + bounds[k - 1] * g.d, (bounds[k] - bounds[k - 1]) * g.d * sizeof(int32_t));
+        if (more) {
+            const int r = finish();  // always ends a launched range, whatever the writes did
+            if (!rc) rc = r;
+        }
+    }
+    if (gen) (void)hipStreamDestroy(gen);
+    ok = std::fclose(f) == 0 && ok;
+    return rc ? rc : (ok ? PPA_OK : PPA_ERR_INVALID);
 }
 
 int cuPPA::printDegreeDistribution(const ExperimentConfiguration &conf)

@@ -37,6 +37,10 @@ public:
 
 private:
     int run(const ExperimentConfiguration &conf, const char *timingCsv, bool adaptive);
+    // Generation in node ranges (ppa_generate_range) with --csr-file written
+    // as they finish: range k's rows are copied out and written while range
+    // k+1 is generated. *total sums the ranges' statistics.
+    int generateStreamingCsr(const ExperimentConfiguration &conf, const ppa_graph_desc &g, ppa_stats *total);
     ppa_graph_desc desc(const ExperimentConfiguration &conf) const;
 
     int64_t _N;

@@ -158,6 +158,18 @@ int ppa_generate(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets,
 int ppa_generate_async(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets,
                        int64_t *d_row_ptr, void *stream);
 int ppa_ctx_sync(ppa_ctx *ctx, void *stream);
+/* No counterpart in the reference (whose writers start after the whole graph,
+ * PAHelper.hpp:163-177): for callers that copy a graph out while the rest is
+ * still being generated. Enqueues, like ppa_generate_async, the generation of
+ * nodes [first, last) only; ppa_ctx_sync ends it. A graph is covered by
+ * consecutive calls: the first has first = 0 (it also fills all of d_targets
+ * and d_row_ptr), each later one starts where the previous one ended, with
+ * the same desc and d_targets, on the same stream. When ppa_ctx_sync returns,
+ * rows [0, last) are final. The ranges give the graph ppa_generate gives.
+ * PPA_ERR_INVALID for last <= first, last > n, or a range that does not
+ * continue the previous one. */
+int ppa_generate_range(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets,
+                       int64_t *d_row_ptr, uint64_t first, uint64_t last, void *stream);
 
 /* ---- degree pass: replaces cuPPA::printDegreeDistribution (cuPPA.cu:103-165)
  *      and PAHelper::outputSCMDegreeDistribution (PAHelper.hpp:15-67) ------- */

@@ -151,3 +151,58 @@ def test_watchdog_turns_a_long_run_into_a_timeout(ctx, oracle_mod):
     want, _, _ = oracle_mod.generate(50_000, d, T, seed)
     got, _ = _gpu(ctx, 50_000, d, T, seed)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n,d,p,seed,cuts", [
+    (200_000, 8, 0.2, 3, [1_000, 50_000, 120_000]),
+    (60_000, 16, 0.1, 4, [500, 30_000]),     # the first range ends inside the LDS prefix
+    (10_000, 4, 0.5, 5, [5]),                # the first range is the seed clique alone
+    (300_000, 1, 0.3, 6, [100_000]),
+])
+def test_generate_in_ranges(ctx, oracle_mod, n, d, p, seed, cuts):
+    """ppa_generate_range: consecutive node ranges give ppa_generate's graph,
+    and after each range's sync the rows below its end are final."""
+    import torch
+    import cuppa
+    T = oracle_mod.p_threshold(p)
+    want, _, _ = oracle_mod.generate(n, d, T, seed)
+    desc = cuppa.make_desc(n, d, None, seed, T)
+    targets = torch.full((n * d,), -7, dtype=torch.int32, device="cuda")
+    rp = torch.full((n + 1,), -7, dtype=torch.int64, device="cuda")
+    bounds = [0] + cuts + [n]
+    for a, b in zip(bounds, bounds[1:]):
+        ctx.generate_range(desc, targets, rp, a, b)
+        ctx.sync()
+        assert ctx.stats()["nodes"] == b - max(a, d + 1), (a, b)
+        assert np.array_equal(targets[:b * d].cpu().numpy(), want[:b * d]), (a, b)
+    assert np.array_equal(rp.cpu().numpy(), np.arange(n + 1, dtype=np.int64) * d)
+
+
+def test_generate_range_must_continue(ctx):
+    """A range that does not start where the previous one ended, or follows
+    a completed graph, is rejected before anything is launched."""
+    import torch
+    import cuppa
+    n, d = 50_000, 6
+    desc = cuppa.make_desc(n, d, 0.3, 9)
+    targets = torch.empty(n * d, dtype=torch.int32, device="cuda")
+    bad = []
+    for first, last in ((0, 0), (10, 5), (0, n + 1)):
+        with pytest.raises(cuppa.PPAError) as e:
+            ctx.generate_range(desc, targets, None, first, last)
+        bad.append(e.value.code)
+    ctx.generate_range(desc, targets, None, 0, 20_000)
+    ctx.sync()
+    with pytest.raises(cuppa.PPAError) as e:       # a gap
+        ctx.generate_range(desc, targets, None, 20_001, n)
+    bad.append(e.value.code)
+    other = cuppa.make_desc(n, d, 0.3, 10)
+    with pytest.raises(cuppa.PPAError) as e:       # another graph
+        ctx.generate_range(other, targets, None, 20_000, n)
+    bad.append(e.value.code)
+    ctx.generate_range(desc, targets, None, 20_000, n)
+    ctx.sync()
+    with pytest.raises(cuppa.PPAError) as e:       # the graph is complete
+        ctx.generate_range(desc, targets, None, n - 1, n)
+    bad.append(e.value.code)
+    assert bad == [-1] * 6

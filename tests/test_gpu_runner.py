@@ -63,6 +63,27 @@ def test_runner_gpu_csr_files(tmp_path, oracle_mod):
     assert np.array_equal(idx, want_src)
 
 
+def test_runner_csr_written_while_generating(tmp_path, oracle_mod):
+    """Above 2^20 nodes `-A cuppa --csr-file` generates the graph in ranges of
+    2^20 nodes and writes each one's rows while the next is generated; the
+    file is the graph's CSR all the same."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_abi import read_csr
+    n, d, p, seed = 2_500_000, 3, 0.4, 21
+    r = subprocess.run([RUNNER, "-A", "cuppa", "-n", str(n), "-x", str(d), "-a", str(p), "--seed", str(seed),
+                        "--csr-file", str(tmp_path / "out.csr")],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Computation Time: " in r.stdout and "Ended" in r.stdout
+    T = oracle_mod.p_threshold(p)
+    G, _, _ = oracle_mod.generate(n, d, T, seed)
+    magic, n2, nnz, d2, kind, seed2, thr, ptr, idx = read_csr(tmp_path / "out.csr")
+    assert (magic, n2, nnz, d2, kind, seed2, thr) == (b"PPACSR1\0", n, n * d, d, 0, seed, T)
+    assert np.array_equal(ptr, np.arange(n + 1, dtype=np.int64) * d)
+    assert np.array_equal(idx, G)
+
+
 def test_runner_adaptive_trials(tmp_path, oracle_mod):
     """`-A cuppa-adaptive` tunes the grid on the graph itself: one CSV row per
     trial launch (the full grid, then 1/8, 1/4, ... of it), and the graph left
