@@ -290,6 +290,41 @@ def serve_trace(count: int, seed: int = 5):
     return [(i, int(n[i]), int(d[i]), thr[float(p[i])], i) for i in range(count)]
 
 
+LATENCY_GRAPHS = [(20_000, 16, 0.1), (100_000, 16, 0.1), (100_000, 16, 0.5), (1_000_000, 16, 0.1),
+                  (10_000_000, 16, 0.1), (100_000_000, 16, 0.1), (100_000, 8, 0.2), (1_000_000, 8, 0.2),
+                  (100_000, 6, 0.1), (100_000, 4, 0.5), (1_000_000, 2, 0.1)]
+
+
+def run_latency(args):
+    """Wall time of one ppa_generate call (inputs resident, synchronised on
+    both sides) per graph of LATENCY_GRAPHS, median of `steps` calls after
+    `warmup`, with the default launch rule and LDS-resident prefix."""
+    import statistics
+    import torch
+    import cuppa
+    ctx = cuppa.Context(0)
+    out = []
+    for n, d, p in LATENCY_GRAPHS:
+        desc = cuppa.make_desc(n, d, p, 3)
+        targets = torch.empty(n * d, dtype=torch.int32, device="cuda")
+        row_ptr = torch.empty(n + 1, dtype=torch.int64, device="cuda")
+        ms, kernel = [], []
+        for i in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.generate_into(desc, targets, row_ptr)
+            if i >= args.warmup:
+                ms.append((time.perf_counter() - t0) * 1e3)
+                kernel.append(ctx.stats()["kernel_ms"])
+        st = ctx.stats()
+        out.append({"n": n, "d": d, "p": p, "call_ms": statistics.median(ms),
+                    "kernel_ms": statistics.median(kernel), "grid_blocks": st["grid_blocks"]})
+        del targets, row_ptr
+    print(json.dumps({"metric": "single-call latency (ppa_generate wall time)", "unit": "ms",
+                      "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "data": "synthetic (keyed Philox draws)", "graphs": out}))
+
+
 def run_serve(args):
     """Served (continuous batching) against sequential plain launches at equal
     p99 latency, under Poisson arrivals."""
@@ -375,9 +410,9 @@ def main():
                          "and the reference's serial BA on a prefix")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB three times (counter calibration) and exit")
-    ap.add_argument("--mode", default="graph", choices=("graph", "batch", "serve"),
+    ap.add_argument("--mode", default="graph", choices=("graph", "batch", "serve", "latency"),
                     help="graph: the single-graph metric (default); batch: the 4096-graph ensemble; "
-                         "serve: the request loop")
+                         "serve: the request loop; latency: single-call wall times of small and mid-size graphs")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--serve-count", type=int, default=100_000)
     ap.add_argument("--serve-plain-count", type=int, default=2000)
@@ -421,6 +456,8 @@ def main():
         return run_batch(args)
     if args.mode == "serve":
         return run_serve(args)
+    if args.mode == "latency":
+        return run_latency(args)
 
     n, d, p, seed = WORKLOADS[args.workload]
     ctx = cuppa.Context(0)
