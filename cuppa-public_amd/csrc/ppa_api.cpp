@@ -75,7 +75,7 @@ int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *ou
 
 uint64_t single_start(const ppa_ctx *ctx, const ppa_graph_desc &desc)
 {
-    return prefix_start(desc.n, desc.d, coin_threshold(desc.model, desc.p_threshold), ctx->tuning.prefix_nodes);
+    return prefix_start(desc.n, desc.d, coin_threshold(desc.model, desc.p_threshold), ctx->tuning.prefix_nodes, true);
 }
 
 uint64_t range_start(const ppa_ctx *ctx, const ppa_graph_desc &desc, uint64_t first, uint64_t last)

@@ -51,7 +51,7 @@ BatchGraph graph_record(const ppa_graph_desc &dsc, uint32_t prefix_cap)
     gr.d = dsc.d;
     gr.model = dsc.model;
     gr.inv_d = 1.0 / (double)dsc.d;
-    gr.start = prefix_start(dsc.n, dsc.d, gr.T, prefix_cap);
+    gr.start = prefix_start(dsc.n, dsc.d, gr.T, prefix_cap, false);
     return gr;
 }
 

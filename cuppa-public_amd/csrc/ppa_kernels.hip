@@ -774,37 +774,37 @@ void generate_kernel(GenArgs a)
 // The workgroup's lanes form groups of GS >= d lanes (GS a power of two, so a
 // group lies inside one 16-lane DPP row); a group owns nodes d+1+g,
 // d+1+g+groups, ... in ascending order, one slot per lane. Attempt 0 of every
-// slot, and attempt 1 for a first duplicate, are drawn before the first pass,
-// in parallel, and parked in the empty slot; on taking a node each slot lane
-// picks them up there. A round of the group compares every
-// slot's candidate with those of the slots before it (DPP row shifts) and
-// finds the first slot from `acc` (the slots accepted so far) on that is
-// still empty or a duplicate: the slots before it are accepted at once; a
-// duplicate there, and every later slot that repeats an accepted one, draws
-// its next attempt and the group goes on; an empty source ends the group's
-// pass. A slot's work is thus spread over the node's lanes
-// instead of a loop over the node's slots on one lane. Passes re-read the
-// still-empty sources and are separated by workgroup barriers, which also
-// make the slots written in a pass visible to the next. The smallest
-// unfinished node only depends on finished slots, so every pass finishes at
-// least one slot.
+// slot (and, in a 64 KB prefix, attempt 1 for a first duplicate) is drawn
+// before the first round, in parallel, and parked in the empty slot; on
+// taking a node each slot lane picks it up there. A round of the group
+// compares every slot's candidate with those of the slots before it (DPP row
+// shifts) and finds the first slot from `acc` (the slots accepted so far) on
+// that is still empty or a duplicate: the slots before it are accepted at
+// once; a duplicate there, and every later slot that repeats an accepted one,
+// draws its next attempt and the group goes on; an empty source ends the
+// group's rounds. A slot's work is thus spread over the node's lanes instead
+// of a loop over the node's slots on one lane. There are no barriers: a
+// wavefront re-reads its empty sources until one fills, and LDS makes a
+// stored slot visible to the others at once. The smallest unfinished node
+// only depends on finished slots, so its group always progresses.
 constexpr int kPrefixThreads = (int)kPrefixLanes;
 constexpr int32_t kNoCandidate = -3;  // DPP fill for lanes outside the row: equals no candidate
-// A parked draw in 15 bits: bit 14 marks a direct link to the node in bits
-// 0..13, else bits 0..13 are the source pair (both below kPrefixSlots).
-constexpr uint32_t kParkedDirect = 1u << 14;
-static_assert(kPrefixSlots <= kParkedDirect, "parked draws must fit 14 bits");
 
+// A parked draw in B+1 bits: bit B marks a direct link to the node in bits
+// 0..B-1, else those bits are the source pair (both below the prefix's
+// slots: a 64 KB prefix parks two attempts of 15 bits, a 160 KB one one of 17).
+template <uint32_t B>
 __device__ __forceinline__ uint32_t park(const Attempt &at, uint32_t d)
 {
-    // pair < P*d <= kPrefixSlots: a 32-bit quotient
-    return at.direct ? kParkedDirect | ((uint32_t)at.pair / d) : (uint32_t)at.pair;
+    // pair < P*d <= the prefix's slots: a 32-bit quotient
+    return at.direct ? (1u << B) | ((uint32_t)at.pair / d) : (uint32_t)at.pair;
 }
 
 // The kernel's register form: ~node (direct, < 0) or the source pair (>= 0).
+template <uint32_t B>
 __device__ __forceinline__ int32_t unpark(uint32_t c)
 {
-    return c & kParkedDirect ? ~(int32_t)(c & (kParkedDirect - 1u)) : (int32_t)c;
+    return c & (1u << B) ? ~(int32_t)(c & ((1u << B) - 1u)) : (int32_t)c;
 }
 
 // The smallest lane of this lane's group, below lane j, that holds v (j if
@@ -836,11 +836,15 @@ __device__ __forceinline__ uint32_t first_same(int32_t v, uint32_t j)
     return first_same_of(v, j, std::make_integer_sequence<int, GS - 1>{});
 }
 
-template <int GS, bool BATCH>
+template <int GS, bool BATCH, uint32_t SLOTS>
 __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
 {
     static_assert(GS >= 1 && GS <= 16 && (GS & (GS - 1)) == 0, "a group lies inside one DPP row");
-    __shared__ int32_t Gs[kPrefixSlots];
+    // two parked attempts of 15 bits, or one of 17
+    constexpr bool kTwo = SLOTS <= (1u << 14);
+    constexpr uint32_t kB = kTwo ? 14u : 16u;
+    static_assert(SLOTS <= (1u << 16), "parked draws must fit 16 bits");
+    __shared__ int32_t Gs[SLOTS];
     __shared__ int32_t s_abort;  // 0, or the error that ends the prefix
     // a batch's graph, or the single graph of the arguments
     uint32_t gi = 0;
@@ -859,9 +863,10 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
     const uint64_t P = a.prefix;
     const uint32_t seed_slots = (d + 1) * d;
     const uint32_t used = (uint32_t)P * d;
-    // Attempts 0 and 1 of every generated slot are drawn up front, by all
-    // lanes at once, and parked in the slot itself as bit 31 | attempt 1 <<
-    // 15 | attempt 0: negative, so readers still find the slot empty.
+    // Attempt 0 of every generated slot (and attempt 1, if kTwo) is drawn up
+    // front, by all lanes at once, and parked in the slot itself as bit 31 |
+    // attempt 1 << 15 | attempt 0 (or bit 31 | attempt 0): negative, so
+    // readers still find the slot empty.
     uint32_t draws = 0;
     for (uint32_t s = threadIdx.x; s < used; s += kPrefixThreads) {
         int32_t v;
@@ -870,8 +875,12 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
         } else {
             const uint32_t tn = s / d, k = s - tn * d;
             const Attempt a0 = attempt(a.key0, a.key1, a.T, tn, k, 0u, (uint64_t)tn * d);
-            const Attempt a1 = attempt(a.key0, a.key1, a.T, tn, k, 1u, (uint64_t)tn * d);
-            v = (int32_t)(0x80000000u | park(a1, d) << 15 | park(a0, d));
+            if constexpr (kTwo) {
+                const Attempt a1 = attempt(a.key0, a.key1, a.T, tn, k, 1u, (uint64_t)tn * d);
+                v = (int32_t)(0x80000000u | park<kB>(a1, d) << 15 | park<kB>(a0, d));
+            } else {
+                v = (int32_t)(0x80000000u | park<kB>(a0, d));
+            }
             ++draws;
         }
         Gs[s] = v;
@@ -897,20 +906,24 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
         return c < 0 ? ~c : __hip_atomic_load(&Gs[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     };
     uint32_t nodes = 0, retries = 0;
-    int32_t spare = 0;  // attempt 1, parked with attempt 0
+    int32_t spare = 0;  // attempt 1, parked with attempt 0 (kTwo)
     auto take = [&]() {  // this lane's slot of node t: attempt 0, from where it is parked
         const uint32_t e =
             (uint32_t)__hip_atomic_load(&Gs[t * d + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        code = unpark(e & 0x7fffu);
-        spare = unpark(e >> 15 & 0x7fffu);
+        if constexpr (kTwo) {
+            code = unpark<kB>(e & 0x7fffu);
+            spare = unpark<kB>(e >> 15 & 0x7fffu);
+        } else {
+            code = unpark<kB>(e & 0x1ffffu);
+        }
         cv = lds_slot(code);
     };
     auto draw = [&]() {  // attempt r >= 1
-        if (r == 1) {
+        if (kTwo && r == 1) {
             code = spare;
         } else {
             const Attempt at = attempt(a.key0, a.key1, a.T, t, j, r, t * d);
-            code = unpark(park(at, d));
+            code = unpark<kB>(park<kB>(at, d));
         }
         cv = lds_slot(code);
         ++draws;
@@ -1103,8 +1116,13 @@ hipError_t launch_generate_d(const GenArgs &a, uint32_t blocks, uint32_t threads
 template <int GS>
 hipError_t launch_prefix_gs(PrefixArgs a, uint32_t d, hipStream_t stream)
 {
+    // the host sized the prefix by prefix_slots(d, true): the 160 KB kernel for 16-lane groups
+    static_assert(GS == 16 || GS < (int)kWidePrefixMinD, "16-lane groups are exactly d >= kWidePrefixMinD");
     a.d = d;
-    hipLaunchKernelGGL((prefix_kernel<GS, false>), dim3(1), dim3(kPrefixThreads), 0, stream, a);
+    if constexpr (GS == 16)
+        hipLaunchKernelGGL((prefix_kernel<GS, false, kPrefixSlotsWide>), dim3(1), dim3(kPrefixThreads), 0, stream, a);
+    else
+        hipLaunchKernelGGL((prefix_kernel<GS, false, kPrefixSlots>), dim3(1), dim3(kPrefixThreads), 0, stream, a);
     return hipGetLastError();
 }
 
@@ -1167,7 +1185,7 @@ hipError_t launch_prefix(uint32_t d, const PrefixArgs &a, hipStream_t stream)
 hipError_t launch_prefix_batch(const PrefixArgs &a, uint32_t count, hipStream_t stream)
 {
     static_assert(kBatchDMax == 16, "batch prefix groups are 16 lanes");
-    hipLaunchKernelGGL((prefix_kernel<16, true>), dim3(count), dim3(kPrefixThreads), 0, stream, a);
+    hipLaunchKernelGGL((prefix_kernel<16, true, kPrefixSlots>), dim3(count), dim3(kPrefixThreads), 0, stream, a);
     return hipGetLastError();
 }
 

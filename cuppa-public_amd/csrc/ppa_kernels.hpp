@@ -96,33 +96,50 @@ struct PrefixArgs {
     uint32_t *done_flag;      // batch: per-graph host flag, raised here for a graph the prefix completes
 };
 
-// int32 slots of the LDS-resident prefix: with its 4-byte abort word, 65,284
-// of the workgroup's 65,536 bytes of LDS. It holds prefix_nodes(n, d) nodes,
-// over the kPrefixLanes lanes of its one workgroup.
+// int32 slots of the LDS-resident prefix, over the kPrefixLanes lanes of its
+// one workgroup. A single graph of d >= kWidePrefixMinD (16-lane groups) may
+// take the workgroup's whole LDS: kPrefixSlotsWide slots and the 4-byte abort
+// word are 160,004 of gfx950's 163,840 bytes. Other prefixes, and every
+// prefix of a batch (whose workgroups then share a CU two by two), stay in
+// 64 KB: kPrefixSlots and the abort word are 65,284 bytes.
 constexpr uint32_t kPrefixSlots = 16320;
+constexpr uint32_t kPrefixSlotsWide = 40000;
+constexpr uint32_t kWidePrefixMinD = 9;
 constexpr uint32_t kPrefixLanes = 1024;
-inline uint64_t prefix_nodes(uint64_t n, uint32_t d)
+inline uint32_t prefix_slots(uint32_t d, bool wide)
 {
-    const uint64_t p = kPrefixSlots / d;
+    return wide && d >= kWidePrefixMinD ? kPrefixSlotsWide : kPrefixSlots;
+}
+inline uint64_t prefix_nodes(uint64_t n, uint32_t d, bool wide)
+{
+    const uint64_t p = prefix_slots(d, wide) / d;
     return n < p ? n : p;
 }
 // First node the persistent generator claims for a graph: the end of its
-// LDS-resident prefix, or d + 1 without one. prefix_cap is the tuning's
-// prefix_nodes: 0 = default (1024 + d + 1 nodes where a node has more than 4
-// copy draws on average, d(1-p) > 4, else none), 1 = none, N = at most N.
-// Measured at n = 2e4 .. 1e7: above 4 copies per node the prefix takes 20-60 %
-// off a call (d = 16, p = 0.1, n = 1e5: 4.0 -> 1.9 ms); at 4 or below it
-// costs 5-15 %, and a longer prefix never helps.
-inline uint64_t prefix_start(uint64_t n, uint32_t d, uint64_t T, uint32_t prefix_cap)
+// LDS-resident prefix, or d + 1 without one. wide: a single graph (the
+// 160 KB prefix is allowed). prefix_cap is the tuning's prefix_nodes:
+// 1 = none, N = at most N, 0 = the default:
+// * none where a node makes at most 4 copy draws on average, d(1-p) <= 4
+//   (one CU would walk nodes the full grid takes in parallel: 5-15 % slower);
+// * as many nodes as LDS holds where a wide prefix is allowed and a node
+//   makes at least 8 copy draws (d = 16: 2,500 nodes took 4-9 % more off
+//   calls of n = 2e4 .. 1e7 than 1,041 nodes did);
+// * else 1024 + d + 1 nodes, one per lane (at d = 8 longer prefixes cost
+//   1-10 %). Above 4 copies per node the prefix takes 25-60 % off a call
+//   (d = 16, p = 0.1, n = 1e5: 4.0 -> 1.8 ms).
+inline uint64_t prefix_start(uint64_t n, uint32_t d, uint64_t T, uint32_t prefix_cap, bool wide)
 {
     if (prefix_cap == 1) return (uint64_t)d + 1;
-    uint64_t p = prefix_nodes(n, d);
+    uint64_t p = prefix_nodes(n, d, wide);
     if (prefix_cap != 0) {
         p = p < prefix_cap ? p : prefix_cap;
     } else {
-        if ((double)d * (1.0 - (double)T / 4294967296.0) <= 4.0) return (uint64_t)d + 1;
-        const uint64_t one_per_lane = (uint64_t)kPrefixLanes + d + 1;
-        p = p < one_per_lane ? p : one_per_lane;
+        const double copies = (double)d * (1.0 - (double)T / 4294967296.0);
+        if (copies <= 4.0) return (uint64_t)d + 1;
+        if (!(wide && d >= kWidePrefixMinD && copies >= 8.0)) {
+            const uint64_t one_per_lane = (uint64_t)kPrefixLanes + d + 1;
+            p = p < one_per_lane ? p : one_per_lane;
+        }
     }
     return p > (uint64_t)d + 1 ? p : (uint64_t)d + 1;
 }

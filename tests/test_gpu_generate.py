@@ -108,9 +108,11 @@ def test_async_launch_owns_the_context(ctx, oracle_mod):
 
 
 @pytest.mark.parametrize("n,d,p,seed", [
-    (1020, 16, 0.1, 1),    # exactly the LDS-resident prefix (16320 / 16 nodes)
-    (1021, 16, 0.1, 2),    # one node for the persistent kernel
-    (2040, 8, 0.5, 3),
+    (2500, 16, 0.1, 1),    # exactly the 160 KB prefix (40000 / 16 nodes)
+    (2501, 16, 0.1, 2),    # one node for the persistent kernel
+    (1020, 16, 0.5, 9),
+    (4444, 9, 0.2, 10),    # d = 9: the smallest d of the 160 KB prefix
+    (2040, 8, 0.5, 3),     # exactly the 64 KB prefix (16320 / 8 nodes)
     (16320, 1, 0.3, 4),    # d = 1: 16320 nodes in LDS, 16 per lane
     (16321, 1, 0.3, 5),
     (3000, 5, 0.0, 6),     # copies only
@@ -119,11 +121,11 @@ def test_async_launch_owns_the_context(ctx, oracle_mod):
 ])
 def test_lds_prefix_boundaries(ctx, oracle_mod, n, d, p, seed):
     """The prefix kernel's nodes and the persistent kernel's continue the same
-    graph: with the prefix as large as LDS allows (16320 / d nodes, up to 16
-    per lane), the default, and none."""
+    graph: with the prefix as large as LDS allows (40000 / d nodes for
+    d >= 9, 16320 / d below), the default, and none."""
     T = oracle_mod.p_threshold(p)
     want, _, want_retries = oracle_mod.generate(n, d, T, seed)
-    for off in (16320, 0, 1):
+    for off in (40000, 0, 1):
         ctx.set_tuning(watchdog_ms=20000, prefix_nodes=off)
         try:
             got, rp = _gpu(ctx, n, d, T, seed)
