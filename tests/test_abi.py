@@ -85,6 +85,10 @@ def test_null_arguments_rejected(L):
     assert L.ppa_check_desc(None) == -1
     assert L.ppa_ctx_destroy(None) == -1
     assert L.ppa_get_stats(None, None) == -1
+    import cuppa
+    desc = cuppa.GraphDesc(100, 4, 0, 0, 0, 0, 0)
+    assert L.ppa_generate_range(None, ctypes.byref(desc), None, None, 0, 50, None) == -1
+    assert L.ppa_generate_async(None, ctypes.byref(desc), None, None, None) == -1
 
 
 def test_ctx_create_reports_missing_gpu(L):
