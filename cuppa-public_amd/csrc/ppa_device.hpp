@@ -10,6 +10,7 @@ constexpr int kWave = 64;
 constexpr uint32_t kMaxRetry = 1u << 24;
 constexpr int kErrRetry = -2;
 constexpr int kErrTimeout = -5;
+constexpr int kErrInternal = -10;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 

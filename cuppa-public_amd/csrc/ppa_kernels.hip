@@ -861,6 +861,11 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
     }
     const uint32_t d = a.d;
     const uint64_t P = a.prefix;
+    if (P * d > SLOTS) {
+        // a prefix sized for another instance: refuse rather than overrun LDS
+        if (threadIdx.x == 0) atomicCAS(&a.scratch->status, 0, kErrInternal);
+        return;
+    }
     const uint32_t seed_slots = (d + 1) * d;
     const uint32_t used = (uint32_t)P * d;
     // Attempt 0 of every generated slot (and attempt 1, if kTwo) is drawn up
@@ -1118,6 +1123,7 @@ hipError_t launch_prefix_gs(PrefixArgs a, uint32_t d, hipStream_t stream)
 {
     // the host sized the prefix by prefix_slots(d, true): the 160 KB kernel for 16-lane groups
     static_assert(GS == 16 || GS < (int)kWidePrefixMinD, "16-lane groups are exactly d >= kWidePrefixMinD");
+    if (a.prefix * d > (GS == 16 ? kPrefixSlotsWide : kPrefixSlots)) return hipErrorInvalidValue;
     a.d = d;
     if constexpr (GS == 16)
         hipLaunchKernelGGL((prefix_kernel<GS, false, kPrefixSlotsWide>), dim3(1), dim3(kPrefixThreads), 0, stream, a);
