@@ -894,7 +894,7 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
     __syncthreads();
 
     constexpr uint32_t kGroups = (uint32_t)kPrefixThreads / GS;
-    constexpr uint32_t kGroupMask = GS == 32 ? 0xffffffffu : (1u << GS) - 1u;
+    constexpr uint32_t kGroupMask = (1u << GS) - 1u;
     const uint32_t j = threadIdx.x % GS;                 // this lane's slot
     const uint32_t gshift = (threadIdx.x & 63u) - j;     // the group's first lane in the wavefront
     const bool slot_lane = j < d;
@@ -903,8 +903,8 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
     bool active = t < P;
     uint32_t acc = 0;
     // This slot's attempt r: code = ~node (direct) or its source pair (< P*d,
-    // so >= 0); cv = its candidate value once known (kEmpty while the source
-    // is still empty).
+    // so >= 0); cv = its candidate value once known, negative while the
+    // source is still empty (kEmpty, or the source's own parked draws).
     uint32_t r = 0;
     int32_t code = 0, cv = kEmpty;
     auto lds_slot = [&](int32_t c) -> int32_t {
