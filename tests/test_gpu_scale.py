@@ -130,3 +130,21 @@ def test_n2p30_sampled_rows_consistent(ctx, big_graph, oracle_mod):
     for start, width in ranges:
         bad = oracle_mod.verify_rows(n, d, T, seed, host, start, width, threads=THREADS)
         assert bad is None, f"row {bad} inconsistent (range start {start})"
+
+
+def test_d16_n2p24_every_row_consistent(ctx, oracle_mod):
+    """d = 16, p = 0.1 at n = 2^24: the 160 KB LDS prefix (2,500 nodes) and the
+    persistent kernel after it. The first 2^20 rows bit for bit against the
+    sequential oracle, every row through the row-consistency verifier."""
+    import torch
+    n, d, p, seed = 1 << 24, 16, 0.1, 5
+    T = oracle_mod.p_threshold(p)
+    print(f"\n[scale] generating n={n} d={d} p={p} seed={seed}", flush=True)
+    g = ctx.generate(n, d, p, seed)
+    host = g.targets.cpu().numpy()
+    assert ctx.stats()["nodes"] == n - d - 1
+    del g
+    torch.cuda.empty_cache()
+    want, _, _ = oracle_mod.generate(1 << 20, d, T, seed)
+    assert np.array_equal(host[:(1 << 20) * d], want)
+    assert oracle_mod.verify_rows(n, d, T, seed, host, threads=THREADS) is None
