@@ -100,11 +100,13 @@ typedef struct ppa_tuning {
     uint32_t sleep_cap;      /* longest back-off of a wavefront whose lanes all wait, in units of
                                 256 clocks, 1..64 (0 = by grid size: 1 for small grids, 32 for a
                                 full GPU) */
-    uint32_t prefix_nodes;   /* LDS-resident prefix: a single graph's first nodes generated by one
+    uint32_t prefix_nodes;   /* LDS-resident prefix: a graph's first nodes generated by one
                                 workgroup with the graph in LDS, where a dependent step costs an
-                                LDS read, not a memory round trip. 0 = default (1024 + d + 1
-                                nodes when d(1-p) > 4, else none), 1 = none, N = at most N
-                                nodes (never more than 16320 / d) */
+                                LDS read, not a memory round trip. 0 = default (none when
+                                d(1-p) <= 4; 40000 / d nodes for a single graph with d >= 9
+                                and d(1-p) >= 8; else 1024 + d + 1), 1 = none, N = at most N
+                                nodes (never more than 40000 / d for a single graph with
+                                d >= 9, 16320 / d otherwise and in batches) */
 } ppa_tuning;
 
 /* Counters of the last generate call (cuPPA_1.cu:84-87 timing CSV and the
