@@ -68,8 +68,10 @@ int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *ou
     const uint64_t waves = g * (th / 64);
     out->sleep_cap = tu.sleep_cap ? tu.sleep_cap : (uint32_t)std::min<uint64_t>(32, std::max<uint64_t>(1, waves / 256));
     // Likewise a lane re-reads the empty sources behind its current slot every
-    // 4th pass in small grids and every 16th in large ones.
-    out->wide_mask = waves >= 4096 ? 15u : 3u;
+    // 2nd pass below 1024 wavefronts, every 4th up to 4096 and every 16th
+    // beyond (every 2nd instead of every 4th: 1-4 % off calls of d = 16 at
+    // n = 2e4 .. 1e6; every pass cost 2 % at n = 1e6, d = 8).
+    out->wide_mask = waves >= 4096 ? 15u : (waves >= 1024 ? 3u : 1u);
     return PPA_OK;
 }
 

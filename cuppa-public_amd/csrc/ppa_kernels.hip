@@ -28,7 +28,7 @@
 //     rejected attempt (duplicate) or an empty source slot costs another
 //     round trip. An empty source is kept in its slot as -2 - source, and a
 //     node's empty sources are re-read together (the first one every pass,
-//     the others every 4th or, in large grids, 16th), so a node waits for its slowest source, not
+//     the others every 2nd, 4th or, in large grids, 16th), so a node waits for its slowest source, not
 //     for the sum of its sources' waits. The reference instead parks
 //     placeholders -(v*M+k) in a per-thread LDS queue and spins
 //     (cuPPA_RRP_1.cu:97-185).
@@ -447,7 +447,7 @@ void generate_kernel(GenArgs a)
         // ---- redraws and the re-reads of still-empty sources (lanes not just assigned) ----
         // One coherent re-read per pass for the source of a redrawn slot k, and
         // for the attempt-0 sources still empty: slot k every pass (batches:
-        // with a back-off), the later ones every 4th or 16th pass. A lane's
+        // with a back-off), the later ones every 2nd, 4th or 16th pass. A lane's
         // re-reads land in rows kk of the LDS info area (a redrawn slot k is
         // never also pending, so its row is free); acceptance stays in slot
         // order, but the waits of a node's slots overlap.
