@@ -3,11 +3,13 @@
 // The reference has no serving layer; this is the request-level form of
 // cuPPA::generateGraph. Output memory comes from one device pool sized by the
 // budget (first-fit free list, 256-byte granules); requests are admitted in
-// arrival order while the pool has room, grouped into launches as they are
-// admitted, and run on several streams at once. Every launch is the batch
-// generator of ppa_batch.cpp with its own scratch, so the graphs it produces
-// are those of ppa_generate. `plain` mode is the baseline: one request per
-// launch on one stream through the single-graph generator.
+// arrival order while the pool has room and run on several streams at once,
+// each launch with its own scratch. A request of solo_nodes or more runs
+// alone on the single-graph generator (with 1/solo_grid_div of the grid a
+// ppa_generate call would use); smaller ones are grouped into launches of the
+// batch generator of ppa_batch.cpp. Either way the graphs are those of
+// ppa_generate. `plain` mode is the baseline: one request per launch on one
+// stream through the single-graph generator with its full grid.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
