@@ -44,6 +44,10 @@
 //   * Below `early_nodes`, where almost every copy source is still being
 //     generated, each slot is published as soon as it is accepted and claims
 //     are one node per lane, which shortens the dependency chains.
+//   * The first 1,000-2,500 nodes of a graph, where the chains are steepest,
+//     come from prefix_kernel: one workgroup with the graph in LDS (see the
+//     LDS-resident prefix section below); the persistent launch then claims
+//     from the prefix's end.
 #include "ppa_device.hpp"
 #include "ppa_kernels.hpp"
 #include "ppa_model.hpp"
