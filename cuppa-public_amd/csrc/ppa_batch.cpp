@@ -215,6 +215,7 @@ int ppa_batch_create(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, 
     for (uint64_t i = 0; i < count; ++i) {
         int rc = ppa_check_desc(&descs[i]);
         if (rc) return rc;
+        if (descs[i].d > kBatchDMax) return PPA_ERR_INVALID;  // the batch generator holds d <= 16
         treg.emplace_back(descs[i].targets_offset, descs[i].n * descs[i].d);
         rreg.emplace_back(descs[i].row_ptr_offset, descs[i].n + 1);
         graphs[i] = graph_record(descs[i], ctx->tuning.prefix_nodes);

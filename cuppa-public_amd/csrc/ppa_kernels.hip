@@ -245,7 +245,8 @@ constexpr int waves_per_simd(int dmax, bool batch)
                  : (dmax <= 5 || dmax == 8) ? 8
                  : (dmax <= 7 || dmax == 12) ? 6
                  : (dmax <= 11 || dmax == 16) ? 5
-                 : 4;
+                 : dmax <= 16 ? 4
+                 : 2;  // d = 17..32: up to 256 VGPRs, no spills
 }
 
 template <int DMAX, bool BATCH>
@@ -775,8 +776,9 @@ void generate_kernel(GenArgs a)
 // the rows out; the persistent generator starts its claims at `prefix`.
 // Same draws, same order of acceptance, hence the same graph.
 //
-// The workgroup's lanes form groups of GS >= d lanes (GS a power of two, so a
-// group lies inside one 16-lane DPP row); a group owns nodes d+1+g,
+// The workgroup's lanes form groups of GS >= d lanes (GS a power of two: a
+// group of up to 16 lanes lies inside one 16-lane DPP row, one of 32 inside
+// half a wavefront); a group owns nodes d+1+g,
 // d+1+g+groups, ... in ascending order, one slot per lane. Attempt 0 of every
 // slot (and, in a 64 KB prefix, attempt 1 for a first duplicate) is drawn
 // before the first round, in parallel, and parked in the empty slot; on
@@ -812,21 +814,28 @@ __device__ __forceinline__ int32_t unpark(uint32_t c)
 }
 
 // The smallest lane of this lane's group, below lane j, that holds v (j if
-// none). row_shr:s hands lane j the value of lane j-s of its 16-lane row
-// (kNoCandidate past the row's start); the GS-1 shifts are independent and
-// their results meet in a tree of minima, keeping the dependent chain short.
-template <int S>
+// none). In groups of up to 16 lanes row_shr:s hands lane j the value of lane
+// j-s of its 16-lane row (kNoCandidate past the row's start); 32-lane groups
+// span two rows and read lane j-s through ds_bpermute instead (lanes j < s
+// read another group's lane and are masked). The GS-1 shifts are
+// independent and their results meet in a tree of minima, keeping the
+// dependent chain short.
+template <int GS, int S>
 __device__ __forceinline__ uint32_t same_at(int32_t v, uint32_t j)
 {
-    const int32_t o = __builtin_amdgcn_update_dpp(kNoCandidate, v, 0x110 + S, 0xf, 0xf, false);
+    int32_t o;
+    if constexpr (GS <= 16)
+        o = __builtin_amdgcn_update_dpp(kNoCandidate, v, 0x110 + S, 0xf, 0xf, false);
+    else
+        o = __builtin_amdgcn_ds_bpermute((int)((lane_id() - (uint32_t)S) << 2), v);
     return j >= (uint32_t)S && o == v ? j - (uint32_t)S : j;
 }
 
-template <int... S>
+template <int GS, int... S>
 __device__ __forceinline__ uint32_t first_same_of(int32_t v, uint32_t j, std::integer_sequence<int, S...>)
 {
     constexpr int N = (int)sizeof...(S) + 1;
-    uint32_t m[N] = {j, same_at<S + 1>(v, j)...};
+    uint32_t m[N] = {j, same_at<GS, S + 1>(v, j)...};
 #pragma unroll
     for (int w = 1; w < N; w *= 2)
 #pragma unroll
@@ -837,13 +846,13 @@ __device__ __forceinline__ uint32_t first_same_of(int32_t v, uint32_t j, std::in
 template <int GS>
 __device__ __forceinline__ uint32_t first_same(int32_t v, uint32_t j)
 {
-    return first_same_of(v, j, std::make_integer_sequence<int, GS - 1>{});
+    return first_same_of<GS>(v, j, std::make_integer_sequence<int, GS - 1>{});
 }
 
 template <int GS, bool BATCH, uint32_t SLOTS>
 __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
 {
-    static_assert(GS >= 1 && GS <= 16 && (GS & (GS - 1)) == 0, "a group lies inside one DPP row");
+    static_assert(GS >= 1 && GS <= 32 && (GS & (GS - 1)) == 0, "a group lies inside half a wavefront");
     // two parked attempts of 15 bits, or one of 17
     constexpr bool kTwo = SLOTS <= (1u << 14);
     constexpr uint32_t kB = kTwo ? 14u : 16u;
@@ -898,7 +907,7 @@ __global__ __launch_bounds__(kPrefixThreads) void prefix_kernel(PrefixArgs a)
     __syncthreads();
 
     constexpr uint32_t kGroups = (uint32_t)kPrefixThreads / GS;
-    constexpr uint32_t kGroupMask = (1u << GS) - 1u;
+    constexpr uint32_t kGroupMask = GS == 32 ? 0xffffffffu : (1u << GS) - 1u;
     const uint32_t j = threadIdx.x % GS;                 // this lane's slot
     const uint32_t gshift = (threadIdx.x & 63u) - j;     // the group's first lane in the wavefront
     const bool slot_lane = j < d;
@@ -1125,11 +1134,11 @@ hipError_t launch_generate_d(const GenArgs &a, uint32_t blocks, uint32_t threads
 template <int GS>
 hipError_t launch_prefix_gs(PrefixArgs a, uint32_t d, hipStream_t stream)
 {
-    // the host sized the prefix by prefix_slots(d, true): the 160 KB kernel for 16-lane groups
-    static_assert(GS == 16 || GS < (int)kWidePrefixMinD, "16-lane groups are exactly d >= kWidePrefixMinD");
-    if (a.prefix * d > (GS == 16 ? kPrefixSlotsWide : kPrefixSlots)) return hipErrorInvalidValue;
+    // the host sized the prefix by prefix_slots(d, true): the 160 KB kernel for 16- and 32-lane groups
+    static_assert(GS >= 16 || GS < (int)kWidePrefixMinD, "16- and 32-lane groups are exactly d >= kWidePrefixMinD");
+    if (a.prefix * d > (GS >= 16 ? kPrefixSlotsWide : kPrefixSlots)) return hipErrorInvalidValue;
     a.d = d;
-    if constexpr (GS == 16)
+    if constexpr (GS >= 16)
         hipLaunchKernelGGL((prefix_kernel<GS, false, kPrefixSlotsWide>), dim3(1), dim3(kPrefixThreads), 0, stream, a);
     else
         hipLaunchKernelGGL((prefix_kernel<GS, false, kPrefixSlots>), dim3(1), dim3(kPrefixThreads), 0, stream, a);
@@ -1167,7 +1176,8 @@ hipError_t launch_fill_batch(const BatchFillArgs &a, uint64_t total_slots, uint6
 }
 
 #define PPA_FOR_EACH_D(X) \
-    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
+    X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32)
 
 hipError_t launch_generate(uint32_t d, const GenArgs &a, uint32_t blocks, uint32_t threads,
                            hipStream_t stream)
@@ -1184,12 +1194,13 @@ hipError_t launch_generate(uint32_t d, const GenArgs &a, uint32_t blocks, uint32
 hipError_t launch_prefix(uint32_t d, const PrefixArgs &a, hipStream_t stream)
 {
     // groups of the next power of two >= d lanes
-    if (d == 0 || d > 16) return hipErrorInvalidValue;
+    if (d == 0 || d > kPrefixMaxD) return hipErrorInvalidValue;
     if (d == 1) return launch_prefix_gs<1>(a, d, stream);
     if (d == 2) return launch_prefix_gs<2>(a, d, stream);
     if (d <= 4) return launch_prefix_gs<4>(a, d, stream);
     if (d <= 8) return launch_prefix_gs<8>(a, d, stream);
-    return launch_prefix_gs<16>(a, d, stream);
+    if (d <= 16) return launch_prefix_gs<16>(a, d, stream);
+    return launch_prefix_gs<32>(a, d, stream);
 }
 
 hipError_t launch_prefix_batch(const PrefixArgs &a, uint32_t count, hipStream_t stream)

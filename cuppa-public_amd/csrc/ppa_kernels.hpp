@@ -105,6 +105,7 @@ struct PrefixArgs {
 constexpr uint32_t kPrefixSlots = 16320;
 constexpr uint32_t kPrefixSlotsWide = 40000;
 constexpr uint32_t kWidePrefixMinD = 9;
+constexpr uint32_t kPrefixMaxD = 32;  // a node's slots fit one half-wavefront group
 constexpr uint32_t kPrefixLanes = 1024;
 inline uint32_t prefix_slots(uint32_t d, bool wide)
 {
@@ -129,7 +130,7 @@ inline uint64_t prefix_nodes(uint64_t n, uint32_t d, bool wide)
 //   (d = 16, p = 0.1, n = 1e5: 4.0 -> 1.8 ms).
 inline uint64_t prefix_start(uint64_t n, uint32_t d, uint64_t T, uint32_t prefix_cap, bool wide)
 {
-    if (prefix_cap == 1) return (uint64_t)d + 1;
+    if (prefix_cap == 1 || d > kPrefixMaxD) return (uint64_t)d + 1;
     uint64_t p = prefix_nodes(n, d, wide);
     if (prefix_cap != 0) {
         p = p < prefix_cap ? p : prefix_cap;

@@ -373,7 +373,8 @@ int ppa_server_step(ppa_server *s)
         uint64_t nodes = 0;
         while (!s->ready.empty() && ids.size() < s->cfg.max_batch) {
             const Request &r = s->reqs[s->ready.front()];
-            const bool solo = s->cfg.plain || r.req.n >= s->cfg.solo_nodes;
+            // (the batch generator holds d <= kBatchDMax: wider requests launch alone)
+            const bool solo = s->cfg.plain || r.req.n >= s->cfg.solo_nodes || r.req.d > kBatchDMax;
             if (solo && !ids.empty()) break;
             ids.push_back(s->ready.front());
             s->ready.pop_front();

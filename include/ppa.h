@@ -53,7 +53,9 @@ enum {
 };
 
 #define PPA_MAX_RETRY (1u << 24)
-#define PPA_MAX_D 16u            /* largest out-degree the generator supports */
+#define PPA_MAX_D 32u            /* largest out-degree the generator supports */
+#define PPA_MAX_BATCH_D 16u      /* largest out-degree of a batch's graphs and of requests the
+                                    server may batch (larger ones always launch alone) */
 #define PPA_MAX_N 0x7fffffffull  /* targets are int32 (DataTypes.h:11-18) */
 
 /* ---- graph descriptor ---------------------------------------------------- */

@@ -66,7 +66,8 @@ def test_strerror_and_threshold(L, oracle_mod):
 
 @pytest.mark.parametrize("n,d,T,ok", [
     (10, 4, 0, True), (5, 4, 0, True), (4, 4, 0, False), (10, 0, 0, False), (100, 16, 0, True),
-    (100, 17, 0, False), (2**31 - 1, 8, 0, True), (2**31, 8, 0, False), (10, 2, 2**32, True),
+    (100, 17, 0, True), (100, 32, 0, True), (100, 33, 0, False), (33, 32, 0, True), (32, 32, 0, False),
+    (2**31 - 1, 8, 0, True), (2**31, 8, 0, False), (10, 2, 2**32, True),
     (10, 2, 2**32 + 1, False)])
 def test_check_desc(L, n, d, T, ok):
     import cuppa
@@ -100,7 +101,8 @@ def test_ctx_create_reports_missing_gpu(L):
 
 
 @pytest.mark.parametrize("n,d,p,seed", [(6, 5, 0.5, 1), (20_000, 1, 0.5, 2), (50_000, 8, 0.2, 11),
-                                        (30_000, 16, 0.9, 3), (40_000, 3, 0.0, 4), (1_000_000, 4, 0.5, 42)])
+                                        (30_000, 16, 0.9, 3), (40_000, 3, 0.0, 4), (1_000_000, 4, 0.5, 42),
+                                        (20_000, 32, 0.3, 5), (5_000, 17, 0.0, 6)])
 def test_cpu_path_matches_oracle(L, oracle_mod, n, d, p, seed):
     import cuppa
     desc = cuppa.make_desc(n, d, p, seed)

@@ -118,6 +118,9 @@ def test_async_launch_owns_the_context(ctx, oracle_mod):
     (3000, 5, 0.0, 6),     # copies only
     (5000, 3, 1.0, 7),     # direct only
     (100_000, 16, 0.1, 8),
+    (1250, 32, 0.1, 11),   # exactly the 160 KB prefix at d = 32 (32-lane groups)
+    (1251, 32, 0.1, 12),
+    (2352, 17, 0.2, 13),   # d = 17: 40000 / 17 nodes
 ])
 def test_lds_prefix_boundaries(ctx, oracle_mod, n, d, p, seed):
     """The prefix kernel's nodes and the persistent kernel's continue the same
