@@ -1,7 +1,8 @@
-"""Out-degrees 17..32 (up to PPA_MAX_D): single graphs on the persistent
-generator (no LDS prefix: a node's slots no longer fit one 16-lane DPP row),
-bit for bit against the oracle; their degree pass, in-edge CSR and ranged
-generation. Batches refuse them, and the server runs them as lone launches."""
+"""Out-degrees 17..32 (up to PPA_MAX_D): single graphs through the LDS prefix
+(32-lane groups, duplicate checks through ds_bpermute) and the persistent
+generator, bit for bit against the oracle; their degree pass, in-edge and
+undirected CSR and ranged generation. Batches refuse them, and the server
+runs them as lone launches."""
 import numpy as np
 import pytest
 
@@ -34,7 +35,7 @@ def test_wide_d_barabasi_albert(ctx, oracle_mod):
     assert np.array_equal(g.targets.cpu().numpy(), want)
 
 
-def test_wide_d_degrees_and_in_csr(ctx, oracle_mod):
+def test_wide_d_degrees_and_csr_forms(ctx, oracle_mod):
     n, d, p, seed = 60_000, 28, 0.4, 8
     T = oracle_mod.p_threshold(p)
     g = ctx.generate(n, d, p_thresh=T, seed=seed)
@@ -48,6 +49,10 @@ def test_wide_d_degrees_and_in_csr(ctx, oracle_mod):
     want_ptr, want_src = oracle_mod.in_csr(n, d, G)
     assert np.array_equal(in_ptr.cpu().numpy(), want_ptr)
     assert np.array_equal(in_src.cpu().numpy(), want_src)
+    ptr, adj = ctx.undirected_csr(g)
+    want_ptr, want_adj = oracle_mod.undirected_csr(n, d, G)
+    assert np.array_equal(ptr.cpu().numpy(), want_ptr)
+    assert np.array_equal(adj.cpu().numpy(), want_adj)
 
 
 def test_wide_d_in_ranges(ctx, oracle_mod):
