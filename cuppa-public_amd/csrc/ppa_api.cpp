@@ -48,7 +48,10 @@ int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *ou
     if (d) {
         const uint64_t dense = (work + 128ull * d - 1) / (128ull * d);  // a lane per 2d nodes
         const uint64_t sparse = (work + 512ull * d - 1) / (512ull * d);  // a lane per 8d nodes
-        waves_needed = std::max<uint64_t>(256, std::min(dense, std::max<uint64_t>(1024, sparse)));
+        // d > 16: a floor of 1024 (5-13 % off calls of n = 1e5 .. 1e6; their
+        // nodes' 17..32 sources make longer waits to overlap)
+        const uint64_t floor_waves = d > 16 ? 1024 : 256;
+        waves_needed = std::max<uint64_t>(floor_waves, std::min(dense, std::max<uint64_t>(1024, sparse)));
     } else {
         waves_needed = (work + ch - 1) / ch;
     }
