@@ -292,7 +292,8 @@ def serve_trace(count: int, seed: int = 5):
 
 LATENCY_GRAPHS = [(20_000, 16, 0.1), (100_000, 16, 0.1), (100_000, 16, 0.5), (1_000_000, 16, 0.1),
                   (10_000_000, 16, 0.1), (100_000_000, 16, 0.1), (100_000, 8, 0.2), (1_000_000, 8, 0.2),
-                  (100_000, 6, 0.1), (100_000, 4, 0.5), (1_000_000, 2, 0.1)]
+                  (100_000, 6, 0.1), (100_000, 4, 0.5), (1_000_000, 2, 0.1), (100_000, 32, 0.1),
+                  (1_000_000, 24, 0.5)]
 
 
 def run_latency(args):

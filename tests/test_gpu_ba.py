@@ -68,11 +68,12 @@ def test_ba_degrees(ctx, oracle_mod, n, d, seed):
     assert int(wdeg.sum()) == 2 * d * (n - d)
 
 
-def test_ba_n2p24_rows_consistent(ctx, oracle_mod):
-    """Every row of a 2^24-node graph is what the model draws from the rows
-    before it (the oracle's row check in copy-model form, pinned to the
+@pytest.mark.parametrize("n,d,seed", [(1 << 24, 8, 5),    # no LDS prefix: 4 copy draws per node
+                                      (1 << 22, 16, 6)])  # the 160 KB prefix: 8 per node
+def test_ba_rows_consistent_at_scale(ctx, oracle_mod, n, d, seed):
+    """Every row of a 2^22..2^24-node graph is what the model draws from the
+    rows before it (the oracle's row check in copy-model form, pinned to the
     endpoint form on the CPU), and the degree sum is 2d(n-d)."""
-    n, d, seed = 1 << 24, 8, 5
     g = _gpu_ba(ctx, n, d, seed)
     G = g.targets.cpu().numpy()
     assert oracle_mod.verify_rows(n, d, 0, seed, G, threads=16, model=1) is None
