@@ -85,9 +85,9 @@ uint64_t single_start(const ppa_ctx *ctx, const ppa_graph_desc &desc)
 
 uint64_t range_start(const ppa_ctx *ctx, const ppa_graph_desc &desc, uint64_t first, uint64_t last)
 {
-    if (first > 0) return first;
+    const uint64_t seeds = (uint64_t)desc.d + 1;  // nodes 0..d are the fill's seed rows
+    if (first > 0) return first > seeds ? first : seeds;
     const uint64_t p = single_start(ctx, desc);
-    const uint64_t seeds = (uint64_t)desc.d + 1;
     return p < last ? p : (last > seeds ? last : seeds);
 }
 

@@ -162,6 +162,7 @@ def test_watchdog_turns_a_long_run_into_a_timeout(ctx, oracle_mod):
     (200_000, 8, 0.2, 3, [1_000, 50_000, 120_000]),
     (60_000, 16, 0.1, 4, [500, 30_000]),     # the first range ends inside the LDS prefix
     (10_000, 4, 0.5, 5, [5]),                # the first range is the seed clique alone
+    (10_000, 8, 0.5, 7, [3, 6, 20]),         # ranges inside the seed clique generate nothing
     (300_000, 1, 0.3, 6, [100_000]),
 ])
 def test_generate_in_ranges(ctx, oracle_mod, n, d, p, seed, cuts):
@@ -178,7 +179,7 @@ def test_generate_in_ranges(ctx, oracle_mod, n, d, p, seed, cuts):
     for a, b in zip(bounds, bounds[1:]):
         ctx.generate_range(desc, targets, rp, a, b)
         ctx.sync()
-        assert ctx.stats()["nodes"] == b - max(a, d + 1), (a, b)
+        assert ctx.stats()["nodes"] == max(0, b - max(a, d + 1)), (a, b)
         assert np.array_equal(targets[:b * d].cpu().numpy(), want[:b * d]), (a, b)
     assert np.array_equal(rp.cpu().numpy(), np.arange(n + 1, dtype=np.int64) * d)
 
