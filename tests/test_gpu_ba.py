@@ -177,3 +177,18 @@ def test_runner_sba(tmp_path, oracle_mod):
     assert edges.shape == ((n - d) * d, 2)
     assert np.array_equal(edges[:, 0], np.repeat(np.arange(d, n), d))
     assert np.array_equal(edges[:, 1], G[d * d:])
+
+
+def test_ba_in_ranges(ctx, oracle_mod):
+    """Barabasi-Albert graphs generated range by range (ppa_generate_range),
+    the first range ending inside the 160 KB prefix (d = 16)."""
+    import torch
+    import cuppa
+    n, d, seed = 30_000, 16, 12
+    want, _ = oracle_mod.generate_ba(n, d, seed)
+    desc = cuppa.make_desc(n, d, seed=seed, model=cuppa.MODEL_BA)
+    targets = torch.full((n * d,), -7, dtype=torch.int32, device="cuda")
+    for a, b in ((0, 700), (700, 9_000), (9_000, n)):
+        ctx.generate_range(desc, targets, None, a, b)
+        ctx.sync()
+        assert np.array_equal(targets[:b * d].cpu().numpy(), want[:b * d]), (a, b)
