@@ -222,8 +222,9 @@ int ppa_undirected_csr(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *
 /* Graph i writes d_targets[descs[i].targets_offset .. + n_i*d_i) and, if
  * d_row_ptr is not NULL, d_row_ptr[descs[i].row_ptr_offset .. + n_i + 1)
  * with values targets_offset_i + t*d_i (offsets into d_targets, so each
- * graph's CSR indexes the concatenated array). Regions must not overlap.
- * Every graph's output equals ppa_generate of that graph alone. */
+ * graph's CSR indexes the concatenated array). Regions must not overlap, and
+ * every d_i <= PPA_MAX_BATCH_D (else PPA_ERR_INVALID). Every graph's output
+ * equals ppa_generate of that graph alone. */
 typedef struct ppa_batch ppa_batch;
 /* Uploads the descriptor table and captures reset + fill + generate into a
  * hipGraph. The batch borrows the context: one launch in flight per context. */
