@@ -23,7 +23,7 @@ bool tuning_valid(const ppa_tuning &t)
     return true;
 }
 
-int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *out)
+int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *out, bool cap_single)
 {
     const ppa_tuning &tu = ctx->tuning;
     if (!tuning_valid(tu)) return PPA_ERR_INVALID;
@@ -33,6 +33,13 @@ int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *ou
     PPA_HIP(generate_occupancy(d, th, &per_cu));
     if (per_cu < 1) per_cu = 1;
     if (tu.blocks_per_cu && (int)tu.blocks_per_cu < per_cu) per_cu = (int)tu.blocks_per_cu;
+    // Single graphs: 16 wavefronts per CU issue the copy reads as fast as the
+    // occupancy limit's 32 (d <= 8) or 20 (d = 16) and keep fewer of them
+    // re-reading still-empty sources. Measured: n = 2^30, d = 8, p = 0.2 in
+    // 174.2 ms against 178.3 ms; n = 2^28, d = 8, p = 0.5 in 29.4 against
+    // 31.1; BA at n = 2^30, d = 8 in 116.3 against 118.6; n = 2^28, d = 2 in
+    // 22.4 against 23.4. 12 per CU lost 5 % at n = 2^27, d = 16, p = 0.9.
+    if (d && cap_single && !tu.blocks_per_cu) per_cu = std::min<int>(per_cu, std::max<int>(1, (int)(kSingleWavesPerCu * 64 / th)));
     uint64_t g = (uint64_t)per_cu * (uint64_t)ctx->num_cus;
     // Single graphs: one lane per 2*d generated nodes, but within 256 and
     // 1024 wavefronts unless one lane per 8*d nodes needs more. Dependency

@@ -40,6 +40,9 @@ constexpr uint32_t kDefaultChunk = 256;
 constexpr uint32_t kDefaultWatchdogMs = 60000;
 constexpr uint32_t kDefaultEarlyNodes = 1u << 22;
 constexpr uint32_t kEarlyChunk = 64;
+// Single-graph generator launches hold at most this many wavefronts per CU
+// unless the tuning's blocks_per_cu says otherwise.
+constexpr uint32_t kSingleWavesPerCu = 16;
 
 class DeviceGuard {
 public:
@@ -74,7 +77,8 @@ struct LaunchShape {
 
 // Grid, workgroup and claim sizes for `work` generated nodes with out-degree
 // d (d = 0: the batch kernel).
-int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *out);
+// cap_single: hold single graphs (d != 0) to kSingleWavesPerCu per CU.
+int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *out, bool cap_single = true);
 // First node the persistent generator claims for a single graph: after the
 // LDS-resident prefix, or d + 1 without it.
 uint64_t single_start(const ppa_ctx *ctx, const ppa_graph_desc &desc);

@@ -185,15 +185,17 @@ int launch(ppa_server *s, Slot &sl, const std::vector<uint64_t> &ids)
         const uint64_t n = graphs[0].n, start = single_start(s->ctx, desc_of(s->reqs[ids[0]].req));
         work = n > start ? n - start : 1;
     }
+    // The server's grid caps below are fractions of the occupancy limit, so
+    // its single launches skip the single-call wavefront cap.
     LaunchShape shape;
-    int rc = resolve_shape(s->ctx, single ? graphs[0].d : 0, work, &shape);
+    int rc = resolve_shape(s->ctx, single ? graphs[0].d : 0, work, &shape, false);
     if (rc) return rc;
     if (!s->cfg.plain) {
         // A persistent launch keeps its workgroups until its work is claimed.
         // Large requests take at most 3/4 of the GPU and batches of small ones
         // at most 1/4, so the two kinds overlap instead of queueing.
         LaunchShape full;
-        if ((rc = resolve_shape(s->ctx, single ? graphs[0].d : 0, ~0ull >> 1, &full)) != PPA_OK) return rc;
+        if ((rc = resolve_shape(s->ctx, single ? graphs[0].d : 0, ~0ull >> 1, &full, false)) != PPA_OK) return rc;
         const uint32_t cap = single && graphs[0].n >= s->cfg.solo_nodes ? full.blocks * 3 / 4 : full.blocks / 4;
         shape.blocks = std::max<uint32_t>(1u, std::min(shape.blocks, cap));
         // A lone request's grid from the single-call rule is sized for its
