@@ -157,7 +157,7 @@ int ensure_work(ppa_ctx *ctx, size_t need, hipStream_t stream)
     return PPA_OK;
 }
 
-int finish_launch(ppa_ctx *ctx, hipStream_t stream)
+static int collect_launch(ppa_ctx *ctx, hipStream_t stream)
 {
     PPA_HIP(hipStreamSynchronize(stream));
     DeviceScratch host{};
@@ -182,6 +182,14 @@ int finish_launch(ppa_ctx *ctx, hipStream_t stream)
     // every generated node must have been finished by exactly one lane
     if (s.nodes != ctx->expect_nodes) return PPA_ERR_INTERNAL;
     return PPA_OK;
+}
+
+int finish_launch(ppa_ctx *ctx, hipStream_t stream)
+{
+    const int rc = collect_launch(ctx, stream);
+    // rows before a failed range's end may be incomplete: no range continues it
+    if (rc != PPA_OK) ctx->range_next = 0;
+    return rc;
 }
 
 }  // namespace ppa

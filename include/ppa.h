@@ -91,8 +91,9 @@ typedef struct ppa_graph_desc {
 
 /* Launch shape of the generator. Any shape gives the same output. */
 typedef struct ppa_tuning {
-    uint32_t block_threads;  /* workgroup size, multiple of 64, <= 1024 (0 = default) */
-    uint32_t blocks_per_cu;  /* resident workgroups per CU (0 = occupancy maximum) */
+    uint32_t block_threads;  /* workgroup size, multiple of 64, <= 256 (0 = default 256) */
+    uint32_t blocks_per_cu;  /* resident workgroups per CU, at most the occupancy maximum (0 = that
+                                maximum, and for ppa_generate* at most 16 wavefronts per CU) */
     uint32_t chunk_nodes;    /* nodes a wavefront claims at once, multiple of 64 (0 = default) */
     uint32_t max_blocks;     /* cap on the persistent grid (0 = no cap) */
     uint32_t watchdog_ms;    /* kernel deadline (0 = default 60000) */
@@ -100,8 +101,8 @@ typedef struct ppa_tuning {
                                 are claimed one per lane (0 = default 2^22, 1 = none); batches
                                 apply at most 16 chunks of it per graph */
     uint32_t sleep_cap;      /* longest back-off of a wavefront whose lanes all wait, in units of
-                                256 clocks, 1..64 (0 = by grid size: 1 for small grids, 32 for a
-                                full GPU) */
+                                256 clocks, 1..64 (0 = by grid size: wavefronts / 256, within
+                                1..32) */
     uint32_t prefix_nodes;   /* LDS-resident prefix: a graph's first nodes generated by one
                                 workgroup with the graph in LDS, where a dependent step costs an
                                 LDS read, not a memory round trip. 0 = default (none when
@@ -171,7 +172,8 @@ int ppa_ctx_sync(ppa_ctx *ctx, void *stream);
  * the same desc and d_targets, on the same stream. When ppa_ctx_sync returns,
  * rows [0, last) are final. The ranges give the graph ppa_generate gives.
  * PPA_ERR_INVALID for last <= first, last > n, or a range that does not
- * continue the previous one. */
+ * continue the previous one. A range whose ppa_ctx_sync fails cannot be
+ * continued: the graph starts again at first = 0. */
 int ppa_generate_range(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets,
                        int64_t *d_row_ptr, uint64_t first, uint64_t last, void *stream);
 

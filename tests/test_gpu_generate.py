@@ -212,3 +212,36 @@ def test_generate_range_must_continue(ctx):
         ctx.generate_range(desc, targets, None, n - 1, n)
     bad.append(e.value.code)
     assert bad == [-1] * 6
+
+
+def test_failed_range_is_not_continued(ctx, oracle_mod):
+    """A range whose kernel ends with an error leaves incomplete rows, so
+    the next range of that graph is refused; the context then generates a
+    new graph, range by range, as the oracle does. The failure is the
+    kernels' own deadline (1 ms against ~25 ms of work), which every
+    wavefront reads within 64 passes, so each kernel exits normally."""
+    import torch
+    import cuppa
+    n, d = 1 << 26, 16
+    desc = cuppa.make_desc(n, d, 0.1, 12)
+    targets = torch.empty(n * d, dtype=torch.int32, device="cuda")
+    ctx.set_tuning(watchdog_ms=1)
+    try:
+        ctx.generate_range(desc, targets, None, 0, n // 2)
+        with pytest.raises(cuppa.PPAError) as e:
+            ctx.sync()
+        assert e.value.code == -5  # PPA_ERR_TIMEOUT
+    finally:
+        ctx.set_tuning(watchdog_ms=20000)
+    with pytest.raises(cuppa.PPAError) as e:
+        ctx.generate_range(desc, targets, None, n // 2, n)
+    assert e.value.code == -1  # PPA_ERR_INVALID
+    del targets
+    m, T, seed = 30_000, oracle_mod.p_threshold(0.3), 13
+    want, _, _ = oracle_mod.generate(m, 8, T, seed)
+    small = cuppa.make_desc(m, 8, None, seed, T)
+    out = torch.full((m * 8,), -7, dtype=torch.int32, device="cuda")
+    for a, b in ((0, 12_000), (12_000, m)):
+        ctx.generate_range(small, out, None, a, b)
+        ctx.sync()
+    assert np.array_equal(out.cpu().numpy(), want)
