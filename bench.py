@@ -209,85 +209,172 @@ def read_requests(profile_dir: str, workload: str):
     return None
 
 
-def batch_specs(count: int = 4096):
-    """The ensemble: n log-uniform in [1e3, 1e7], d uniform in 1..16, p uniform
-    in [0.05, 0.95], Philox seed 1000 + i; the meta-draw is numpy's Philox
-    generator keyed 4096."""
+def _pct(xs, q):
     import numpy as np
-    import cuppa
-    rng = np.random.Generator(np.random.Philox(key=4096))
-    out = []
-    for i in range(count):
-        n = int(round(10 ** rng.uniform(3, 7)))
-        d = int(rng.integers(1, 17))
-        p = float(rng.uniform(0.05, 0.95))
-        out.append((max(n, d + 1), d, cuppa.p_threshold(p), 1000 + i))
-    return out
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
 
 
-def run_batch(args):
-    """4096 independent graphs in one captured persistent launch, against the
-    same graphs generated by 4096 plain ppa_generate calls."""
+def batch_object(args, ctx):
+    """The ensemble in one captured persistent launch against the same graphs
+    generated by plain ppa_generate calls, one after another. Both sides are
+    the median of `runs` full passes after a warm-up. Latency is completion
+    time with the whole ensemble submitted at once: per graph, from the launch
+    to its completion flag (batched) or to the end of its own call (plain),
+    plus the plain calls' own durations."""
+    import statistics
     import torch
     import cuppa
-    specs = batch_specs(args.batch_count)
+    from cuppa import workloads
+    runs = max(3, args.batch_runs)
+    specs = workloads.batch_specs(args.batch_count)
     descs, tot, rot = cuppa.batch_layout(specs)
     edges = sum(n * d for n, d, _, _ in specs)
-    ctx = cuppa.Context(0)
     targets = torch.empty(tot, dtype=torch.int32, device="cuda")
     row_ptr = torch.empty(rot, dtype=torch.int64, device="cuda")
-    # autotune on the ensemble itself (~20 timed launches), then keep the shape:
-    # a sample's best shape is not the whole ensemble's
-    tuned = cuppa.autotune(ctx, descs, targets, row_ptr)
+    tuned = cuppa.autotune(ctx, descs, targets, row_ptr) if args.batch_autotune else ctx.get_tuning()
     b = cuppa.Batch(ctx, descs, targets, row_ptr)
-    b.run()  # warm-up
-    times = []
-    for _ in range(args.steps):
+    b.run()  # warm-up: also the first touch of the output buffers
+    times, dones, kernel = [], [], []
+    for _ in range(runs):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         b.run()
         times.append(time.perf_counter() - t0)
+        dones.append(b.completion_ms())
+        kernel.append(ctx.stats()["kernel_ms"])
     st = ctx.stats()
     b.close()
-    batched = min(times)
-    # plain: one ppa_generate per graph, sequentially, into a scratch buffer
+    del b, targets, row_ptr
+    torch.cuda.empty_cache()
+    order = sorted(range(runs), key=lambda i: times[i])
+    med = order[runs // 2]
+    batched = times[med]
+    done = dones[med]
+    # plain: one ppa_generate per graph into reused buffers, with the default
+    # launch rules (the single-graph tuning)
+    ctx.set_tuning()
     maxn = max(n for n, _, _, _ in specs)
     scratch = torch.empty(maxn * 16, dtype=torch.int32, device="cuda")
     scratch_rp = torch.empty(maxn + 1, dtype=torch.int64, device="cuda")
-    ctx.set_tuning()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for n, d, T, seed in specs:
-        ctx.generate_into(cuppa.GraphDesc(n, d, 0, T, seed, 0, 0), scratch, scratch_rp)
-    torch.cuda.synchronize()
-    plain = time.perf_counter() - t0
-    # the reference's serial path on every 256th graph of the ensemble (~10 s of one core)
+    plain_descs = [cuppa.GraphDesc(n, d, 0, T, seed, 0, 0) for n, d, T, seed in specs]
+    for ds in plain_descs[:64]:  # warm-up
+        ctx.generate_into(ds, scratch, scratch_rp)
+    passes = []
+    for _ in range(runs):
+        torch.cuda.synchronize()
+        per_call = []
+        t_start = time.perf_counter()
+        for ds in plain_descs:
+            t0 = time.perf_counter()
+            ctx.generate_into(ds, scratch, scratch_rp)  # waits for its stream
+            per_call.append(time.perf_counter() - t0)
+        torch.cuda.synchronize()
+        passes.append((time.perf_counter() - t_start, per_call))
+    del scratch, scratch_rp
+    torch.cuda.empty_cache()
+    passes.sort(key=lambda x: x[0])
+    plain, per_call = passes[runs // 2]
+    import numpy as np
+    completion_plain = np.cumsum(per_call) * 1e3
     cpu = cpu_mix_baseline([(n, d, T / 2**32) for n, d, T, _ in specs[::256]], "ensemble") \
         if args.cpu_prefix else None
-    print(json.dumps({
-        "metric": "batched ensemble edges/s (one hipGraph launch) vs plain per-graph launches",
-        "value": edges / batched, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
-        "higher_is_better": True, "dtype": "int32", "data": "synthetic (keyed Philox draws)",
-        "config": {"workload": f"{len(specs)} copy-model graphs, n log-uniform [1e3,1e7], d 1..16, "
-                               f"p [0.05,0.95], seed 1000+i", "graphs": len(specs), "edges": edges},
-        "batched_ms": batched * 1e3, "plain_ms": plain * 1e3, "plain_edges_per_s": edges / plain,
-        "speedup_vs_plain": plain / batched, "tuning": tuned, "cpu_baseline": cpu,
+    return {
+        "workload": f"{len(specs)} copy-model graphs, n log-uniform [1e3,1e7], d 1..16, p [0.05,0.95], "
+                    f"seed 1000+i", "graphs": len(specs), "edges": edges,
+        "batched_ms": batched * 1e3, "batched_runs_ms": [t * 1e3 for t in times],
+        "batched_kernel_ms": kernel[med], "edges_per_s": edges / batched,
+        "plain_ms": plain * 1e3, "plain_runs_ms": sorted(p[0] * 1e3 for p in passes),
+        "plain_edges_per_s": edges / plain,
+        "throughput_ratio": plain / batched,
+        "completion_latency_all_submitted_at_once": {
+            "batched_p50_ms": _pct(done, 50), "batched_p99_ms": _pct(done, 99), "batched_max_ms": max(done),
+            "plain_p50_ms": _pct(completion_plain, 50), "plain_p99_ms": _pct(completion_plain, 99),
+            "plain_max_ms": float(completion_plain[-1])},
+        "plain_per_call_ms": {"p50": _pct(per_call, 50) * 1e3, "p99": _pct(per_call, 99) * 1e3,
+                              "mean": statistics.mean(per_call) * 1e3},
+        "runs": runs, "tuning": tuned, "cpu_baseline": cpu,
         "stats": {k: st[k] for k in ("nodes", "draws", "retries", "prefetch_misses", "polls", "staged_rows",
                                      "grid_blocks", "block_threads")},
-    }))
+    }
 
 
-def serve_trace(count: int, seed: int = 5):
-    """Request sizes: n lognormal (median 1e5, sigma 1.5) clipped to [1e2, 1e8],
-    d in {2,4,8,16}, p in {0.1,0.5,0.9}, Philox seed = request id."""
-    import numpy as np
+def run_batch(args):
     import cuppa
-    rng = np.random.Generator(np.random.Philox(key=seed))
-    n = np.clip(np.rint(rng.lognormal(np.log(1e5), 1.5, count)), 1e2, 1e8).astype(np.int64)
-    d = rng.choice([2, 4, 8, 16], count)
-    p = rng.choice([0.1, 0.5, 0.9], count)
-    thr = {q: cuppa.p_threshold(q) for q in (0.1, 0.5, 0.9)}
-    return [(i, int(n[i]), int(d[i]), thr[float(p[i])], i) for i in range(count)]
+    ctx = cuppa.Context(0)
+    obj = batch_object(args, ctx)
+    print(json.dumps({"metric": "batched ensemble edges/s (one hipGraph launch) vs plain per-graph launches",
+                      "value": obj["edges_per_s"], "unit": "edges/s", "n_gpus": 1, "higher_is_better": True,
+                      "dtype": "int32", "data": "synthetic (keyed Philox draws)", "batch": obj}))
+
+
+def serve_object(args, ctx):
+    """Served (continuous batching) against sequential plain launches at equal
+    p99 latency, under Poisson arrivals, over the same `serve_count` requests
+    on both sides: the plain loop (one request per launch, one stream) at
+    saturation, then at 0.7x its saturation rate (the reference p50/p99); the
+    served loop at multiples of that rate."""
+    import numpy as np
+    import torch
+    import cuppa
+    from cuppa import workloads
+    torch.cuda.empty_cache()  # the pool below is allocated outside torch's cache
+    trace = workloads.serve_trace(args.serve_count)
+    edges_all = float(sum(n * d for _, n, d, _, _ in trace))
+    budget = args.serve_budget_gb << 30
+    sizes = np.array([n for _, n, _, _, _ in trace])
+
+    def replay(plain, rate):
+        arr = workloads.poisson_arrivals(len(trace), rate)
+        srv = cuppa.Server(ctx, budget_bytes=budget, streams=1 if plain else args.serve_streams, plain=plain,
+                           solo_nodes=0 if plain else args.serve_solo_nodes,
+                           batch_nodes=0 if plain else args.serve_batch_nodes,
+                           solo_grid_div=0 if plain else args.serve_solo_grid_div)
+        lat, makespan = srv.replay(trace, arr)
+        srv.close()
+        lat = np.array(lat)
+        by_size = {}
+        for lo, hi, name in ((0, 1e5, "n<1e5"), (1e5, 1e6, "1e5<=n<1e6"), (1e6, 1e7, "1e6<=n<1e7"),
+                             (1e7, 1e9, "n>=1e7")):
+            m = (sizes >= lo) & (sizes < hi)
+            if m.any():
+                by_size[name] = {"count": int(m.sum()), "p50_ms": _pct(lat[m], 50), "p99_ms": _pct(lat[m], 99)}
+        return {"rate_per_s": rate, "requests": len(trace), "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
+                "max_ms": float(lat.max()), "makespan_ms": makespan,
+                "req_per_s": len(trace) / (makespan * 1e-3), "edges_per_s": edges_all / (makespan * 1e-3),
+                "by_size": by_size}
+
+    sat = replay(True, 0)
+    plain_rate = 0.7 * sat["req_per_s"]
+    plain = replay(True, plain_rate)
+    served = []
+    for mult in [float(m) for m in args.serve_mults.split(",")]:
+        res = replay(False, plain_rate * mult)
+        res["rate_multiple"] = mult
+        served.append(res)
+    ok = [s for s in served if s["p99_ms"] <= plain["p99_ms"]]
+    best = max(ok, key=lambda s: s["edges_per_s"]) if ok else None
+    return {
+        "workload": "requests: n lognormal(median 1e5, sigma 1.5) clipped [1e2,1e8], d {2,4,8,16}, "
+                    "p {0.1,0.5,0.9}, seed = id; Poisson arrivals", "requests": len(trace),
+        "edges": edges_all, "budget_gb": args.serve_budget_gb, "streams": args.serve_streams,
+        "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+        "plain_saturation": sat, "plain": plain, "served": served,
+        "best_served_rate_multiple": best["rate_multiple"] if best else None,
+        "throughput_ratio_at_equal_p99": (best["edges_per_s"] / plain["edges_per_s"]) if best else None,
+        # the reference's serial path on the first 64 requests of the trace (one core)
+        "cpu_baseline": cpu_mix_baseline([(n, d, T / 2**32) for _, n, d, T, _ in trace[:64]], "request mix")
+        if args.cpu_prefix else None,
+    }
+
+
+def run_serve(args):
+    import cuppa
+    ctx = cuppa.Context(0)
+    obj = serve_object(args, ctx)
+    best = obj["throughput_ratio_at_equal_p99"]
+    print(json.dumps({"metric": "served throughput at equal p99 latency vs sequential plain launches",
+                      "value": best, "unit": "x plain edges/s", "n_gpus": 1, "higher_is_better": True,
+                      "dtype": "int32", "data": "synthetic (keyed Philox draws, Poisson arrivals)", "serve": obj}))
 
 
 LATENCY_GRAPHS = [(20_000, 16, 0.1), (100_000, 16, 0.1), (100_000, 16, 0.5), (1_000_000, 16, 0.1),
@@ -326,72 +413,57 @@ def run_latency(args):
                       "data": "synthetic (keyed Philox draws)", "graphs": out}))
 
 
-def run_serve(args):
-    """Served (continuous batching) against sequential plain launches at equal
-    p99 latency, under Poisson arrivals."""
-    import numpy as np
+def degree_and_csr(ctx, n, d, edges, targets, row_ptr) -> dict:
+    """The degree pass and the in-edge CSR of the generated graph: the first
+    call of each also allocates the context's work memory; the second is
+    timed. Algorithmic bytes of the degree pass: 4 B/slot read + 8 B/slot
+    degree update + 4 B/node histogram read; of the in-edge CSR: 4 B/slot
+    read + 4 B/slot in_src write + 8 B/node in_ptr write."""
+    import torch
     import cuppa
-    trace = serve_trace(args.serve_count)
-    edges_all = np.array([n * d for _, n, d, _, _ in trace], dtype=np.float64)
-    ctx = cuppa.Context(0)
-    budget = args.serve_budget_gb << 30
+    out = {}
+    g = cuppa.Graph(n, d, targets, row_ptr)
+    times = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        hist, mx, _ = ctx.degree_histogram(g, hist_len=1 << 26)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t1) * 1e3)
+    del hist
+    alg = 12.0 * edges + 4.0 * n
+    out["degree"] = {"ms": times[1], "first_call_ms": times[0], "max_degree": mx,
+                     "algorithmic_gbs": alg / (times[1] * 1e-3) / 1e9,
+                     "roofline_frac": alg / (times[1] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    times = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        in_ptr, in_src = ctx.in_csr(g)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t1) * 1e3)
+        del in_ptr, in_src
+    alg = 8.0 * edges + 8.0 * n
+    out["in_csr"] = {"ms": times[1], "first_call_ms": times[0], "edges_per_s": edges / (times[1] * 1e-3),
+                     "algorithmic_gbs": alg / (times[1] * 1e-3) / 1e9,
+                     "roofline_frac": alg / (times[1] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    ctx.trim()  # give the work memory back before the next measurements
+    torch.cuda.empty_cache()
+    return out
 
-    def replay(plain, reqs, rate, arrivals_seed=1):
-        rng = np.random.Generator(np.random.Philox(key=arrivals_seed))
-        arr = np.cumsum(rng.exponential(1e3 / rate, len(reqs))) if rate else np.zeros(len(reqs))
-        srv = cuppa.Server(ctx, budget_bytes=budget, streams=1 if plain else args.serve_streams, plain=plain,
-                           solo_nodes=0 if plain else args.serve_solo_nodes,
-                           batch_nodes=0 if plain else args.serve_batch_nodes,
-                           solo_grid_div=0 if plain else args.serve_solo_grid_div)
-        lat, makespan = srv.replay(reqs, arr.tolist())
-        srv.close()
-        lat = np.array(lat)
-        e = edges_all[:len(reqs)].sum()
-        sizes = np.array([n for _, n, _, _, _ in reqs])
-        by_size = {}
-        for lo, hi, name in ((0, 1e5, "n<1e5"), (1e5, 1e6, "1e5<=n<1e6"), (1e6, 1e7, "1e6<=n<1e7"), (1e7, 1e9, "n>=1e7")):
-            m = (sizes >= lo) & (sizes < hi)
-            if m.any():
-                by_size[name] = {"count": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50)),
-                                 "p99_ms": float(np.percentile(lat[m], 99))}
-        return {"rate_per_s": rate, "requests": len(reqs), "p50_ms": float(np.percentile(lat, 50)),
-                "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()), "makespan_ms": makespan,
-                "req_per_s": len(reqs) / (makespan * 1e-3), "edges_per_s": e / (makespan * 1e-3),
-                "by_size": by_size}
 
-    # plain: saturation throughput on a prefix, then 0.7x of it
-    np_plain = min(args.serve_plain_count, len(trace))
-    sat = replay(True, trace[:np_plain], 0)
-    plain_rate = 0.7 * sat["req_per_s"]
-    plain = replay(True, trace[:np_plain], plain_rate)
-    # served: increasing arrival rates; each run covers at most ~10 s of arrivals
-    served = []
-    for mult in [float(m) for m in args.serve_mults.split(",")]:
-        rate = plain_rate * mult
-        count = int(min(len(trace), max(np_plain, rate * 10)))
-        res = replay(False, trace[:count], rate)
-        res["rate_multiple"] = mult
-        served.append(res)
-        if res["p99_ms"] > plain["p99_ms"] and mult > 1:
-            break
-    ok = [s for s in served if s["p99_ms"] <= plain["p99_ms"]]
-    best = max(ok, key=lambda s: s["edges_per_s"]) if ok else None
-    print(json.dumps({
-        "metric": "served throughput at equal p99 latency vs sequential plain launches",
-        "value": best["edges_per_s"] if best else None, "unit": "edges/s", "n_gpus": 1,
-        "higher_is_better": True, "dtype": "int32", "data": "synthetic (keyed Philox draws, Poisson arrivals)",
-        "config": {"workload": "requests: n lognormal(median 1e5, sigma 1.5) clipped [1e2,1e8], d {2,4,8,16}, "
-                               "p {0.1,0.5,0.9}, seed = id", "requests": len(trace),
-                   "budget_gb": args.serve_budget_gb, "streams": args.serve_streams,
-                   "solo_nodes": args.serve_solo_nodes, "batch_nodes": args.serve_batch_nodes,
-                   "solo_grid_div": args.serve_solo_grid_div,
-                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
-        "plain_saturation": sat, "plain": plain, "served": served,
-        "throughput_ratio_at_equal_p99": (best["edges_per_s"] / plain["edges_per_s"]) if best else None,
-        # the reference's serial path on the first 64 requests of the trace (one core)
-        "cpu_baseline": cpu_mix_baseline([(n, d, T / 2**32) for _, n, d, T, _ in trace[:64]], "request mix")
-        if args.cpu_prefix else None,
-    }))
+def profile_kernel_ms(profile_dir: str, workload: str):
+    """The generation kernel's average duration in the committed PMC summary
+    (the run its traffic figures come from), or None."""
+    path = os.path.join(profile_dir, f"generate_kernel_{workload}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            m = re.match(r"\s*generate_kernel_avg_ms\s*[:=]\s*([0-9.eE+]+)", line)
+            if m:
+                return float(m.group(1))
+    return None
 
 
 def main():
@@ -401,9 +473,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="n2p30_d8_p0.2", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-prefix", type=int, default=1 << 25,
-                    help="nodes of the CPU baseline sample (0 = skip)")
-    ap.add_argument("--degree", action="store_true", help="also time the degree-histogram pass")
-    ap.add_argument("--in-csr", action="store_true", help="also time the in-edge CSR build")
+                    help="nodes of the CPU baseline sample (0 = skip every CPU baseline)")
+    ap.add_argument("--no-degree", action="store_true", help="skip the degree pass and in-edge CSR timings")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble object")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-loop object")
     ap.add_argument("--no-gather-ceiling", action="store_true",
                     help="skip the random-gather ceiling measurement (needs 35 GB of free HBM)")
     ap.add_argument("--ba", action="store_true",
@@ -412,15 +485,18 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB three times (counter calibration) and exit")
     ap.add_argument("--mode", default="graph", choices=("graph", "batch", "serve", "latency"),
-                    help="graph: the single-graph metric (default); batch: the 4096-graph ensemble; "
-                         "serve: the request loop; latency: single-call wall times of small and mid-size graphs")
+                    help="graph: the single-graph metric with the batch and serve objects (default); "
+                         "batch / serve: one of those objects alone; latency: single-call wall times of "
+                         "small and mid-size graphs")
     ap.add_argument("--batch-count", type=int, default=4096)
-    ap.add_argument("--serve-count", type=int, default=100_000)
-    ap.add_argument("--serve-plain-count", type=int, default=2000)
+    ap.add_argument("--batch-runs", type=int, default=3, help="timed passes of each side (median, >= 3)")
+    ap.add_argument("--batch-autotune", action="store_true",
+                    help="run the autotuner on the ensemble first (about 25 extra launches)")
+    ap.add_argument("--serve-count", type=int, default=10_000,
+                    help="requests of each replay, plain and served alike")
     ap.add_argument("--serve-budget-gb", type=int, default=200)
-    ap.add_argument("--serve-mults", default="1,2,3,4,5,6,7,8,9,10,12,14,16,20,24,32",
-                    help="served arrival rates, as multiples of the plain rate, tried in order until the "
-                         "served p99 exceeds the plain p99")
+    ap.add_argument("--serve-mults", default="5,6,7,8",
+                    help="served arrival rates, as multiples of the plain loop's rate (0.7x its saturation)")
     ap.add_argument("--serve-solo-nodes", type=int, default=0,
                     help="requests of at least this many nodes launch alone (0 = the server's default)")
     ap.add_argument("--serve-batch-nodes", type=int, default=0,
@@ -435,7 +511,7 @@ def main():
         print(f"bench.py: only --gpus 1 is supported; this generator runs on one GPU "
               f"(asked for {args.gpus})", file=sys.stderr)
         sys.exit(2)
-    if args.mode == "serve":
+    if args.mode == "serve" or (args.mode == "graph" and not args.no_serve):
         # Streams beyond the runtime's hardware queues (HIP default 4) share a
         # queue and run one after another; this must be set before HIP starts.
         os.environ["GPU_MAX_HW_QUEUES"] = str(max(int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
@@ -490,41 +566,17 @@ def main():
     k_ms = sum(kernel_ms) / len(kernel_ms)
     c_ms = sum(call_ms) / len(call_ms)
     achieved = bpe["generate_kernel"] * edges / (k_ms * 1e-3) / 1e9
-    traffic = read_traffic(os.path.join(ROOT, "profiles"), args.workload)
+    profiles = os.path.join(ROOT, "profiles")
+    traffic = read_traffic(profiles, args.workload)
+    prof_ms = profile_kernel_ms(profiles, args.workload)
+    # the committed counters describe this kernel only if their run's kernel
+    # time matches this run's (within 3 %)
+    traffic_current = prof_ms is not None and abs(prof_ms - k_ms) <= 0.03 * k_ms
     last = stats[-1]
 
     extra = {}
-    if args.degree:
-        # the first call also allocates the pass's buffers; the second is timed
-        g = cuppa.Graph(n, d, targets, row_ptr)
-        times = []
-        for _ in range(2):
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            hist, mx, _ = ctx.degree_histogram(g, hist_len=1 << 26)
-            torch.cuda.synchronize()
-            times.append((time.perf_counter() - t1) * 1e3)
-        # algorithmic bytes: 4 B/slot read + 8 B/slot degree update + 4 B/node histogram read
-        alg = 12.0 * edges + 4.0 * n
-        extra["degree_pass_ms"] = times[1]
-        extra["degree_pass_first_call_ms"] = times[0]
-        extra["degree_pass_algorithmic_gbs"] = alg / (times[1] * 1e-3) / 1e9
-        extra["degree_pass_roofline_frac"] = alg / (times[1] * 1e-3) / 1e9 / HBM_PEAK_GBS
-        extra["max_degree"] = mx
-    if args.in_csr:
-        # the first call also allocates the work area; the second is timed
-        g = cuppa.Graph(n, d, targets, row_ptr)
-        times = []
-        for _ in range(2):
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            in_ptr, in_src = ctx.in_csr(g)
-            torch.cuda.synchronize()
-            times.append((time.perf_counter() - t1) * 1e3)
-            del in_ptr, in_src
-        extra["in_csr_ms"] = times[1]
-        extra["in_csr_first_call_ms"] = times[0]
-        extra["in_csr_edges_per_s"] = edges / (times[1] * 1e-3)
+    if not args.no_degree:
+        extra.update(degree_and_csr(ctx, n, d, edges, targets, row_ptr))
     if args.ba:
         # same n, d and buffers; the first call is the warm-up
         ba_desc = cuppa.make_desc(n, d, seed=seed, model=cuppa.MODEL_BA)
@@ -552,25 +604,30 @@ def main():
                                   "kind": "reference", "seconds": secs,
                                   "sample": f"reference SerialBA (MT19937-64), first {args.cpu_prefix} nodes, d={d}"}
         extra["ba"] = ba
-    copy_gbs = copy_bandwidth_gbs(torch)
 
     del targets, row_ptr
     torch.cuda.empty_cache()
-    requests = read_requests(os.path.join(ROOT, "profiles"), args.workload)
+    copy_gbs = copy_bandwidth_gbs(torch)
+    requests = read_requests(profiles, args.workload)
     request_roof = None
     if requests and not args.no_gather_ceiling:
         ceil = gather_ceiling(torch)
         rate = requests / (k_ms * 1e-3)
         request_roof = {"read_requests_per_launch": requests, "achieved_requests_per_s": rate,
                         "gather_ceiling": ceil, "frac": rate / ceil["requests_per_s"],
-                        "unit": "64-byte read requests/s"}
+                        "unit": "64-byte read requests/s", "requests_from_committed_profile": True}
         if "probe_lds_nt_with_row_stores" in ceil:
             # against reads that share the memory with row stores, as the generator's do
             request_roof["frac_of_read_write_mix"] = rate / ceil["probe_lds_nt_with_row_stores"]
     cpu = cpu_baseline(n, d, p, args.cpu_prefix) if args.cpu_prefix else None
+    if not args.no_batch:
+        extra["batch"] = batch_object(args, ctx)
+        ctx.set_tuning()
+    if not args.no_serve:
+        extra["serve"] = serve_object(args, ctx)
 
     line = {
-        "metric": "edges/s at n=2^30 d=8 + HBM-roofline %",
+        "metric": "edges/s at n=2^30 d=8 + HBM-roofline %; batched-vs-plain p50/p99 latency",
         "value": value,
         "unit": "edges/s",
         "n_gpus": 1,
@@ -585,7 +642,10 @@ def main():
         "config": {"workload": f"copy-model single graph n={n} d={d} p={p} seed={seed}",
                    "n": n, "d": d, "p": p, "seed": seed, "edges": edges},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic if traffic_current else None,
+                     "traffic_profile": {"file": f"profiles/generate_kernel_{args.workload}_pmc.txt",
+                                         "bytes_per_launch": traffic, "kernel_ms": prof_ms,
+                                         "matches_this_run": traffic_current},
                      "kernel": f"generate_kernel<{d}>",
                      "algorithmic_bytes_per_edge": bpe["generate_kernel"],
                      "kernel_ms": k_ms},

@@ -141,9 +141,16 @@ hipError_t enqueue_single(const ppa_ctx *ctx, const ppa_graph_desc &desc, int32_
     return hipSuccess;
 }
 
+uint64_t held_bytes(const ppa_ctx *ctx)
+{
+    return (uint64_t)ctx->work_bytes + ctx->degree_scratch_len * sizeof(uint32_t);
+}
+
 int ensure_work(ppa_ctx *ctx, size_t need, hipStream_t stream)
 {
     if (ctx->work_bytes >= need) return PPA_OK;
+    if (ctx->scratch_budget && held_bytes(ctx) - ctx->work_bytes + need > ctx->scratch_budget)
+        return PPA_ERR_BUDGET;
     PPA_HIP(hipStreamSynchronize(stream));
     if (ctx->work) PPA_HIP(hipFree(ctx->work));
     ctx->work = nullptr;
@@ -157,9 +164,14 @@ int ensure_work(ppa_ctx *ctx, size_t need, hipStream_t stream)
     return PPA_OK;
 }
 
-static int collect_launch(ppa_ctx *ctx, hipStream_t stream)
+// Waits for the pending launch through its own end event (ev[2], recorded on
+// the launch's stream after its last kernel), whatever stream the caller
+// names, and reads its status and counters. The context is free again on
+// every return path.
+static int collect_launch(ppa_ctx *ctx)
 {
-    PPA_HIP(hipStreamSynchronize(stream));
+    ctx->pending = false;
+    PPA_HIP(hipEventSynchronize(ctx->ev[2]));
     DeviceScratch host{};
     PPA_HIP(hipMemcpy(&host, ctx->scratch, sizeof(host), hipMemcpyDeviceToHost));
     ppa_stats &s = ctx->last;
@@ -177,16 +189,15 @@ static int collect_launch(ppa_ctx *ctx, hipStream_t stream)
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) s.kernel_ms = ms;
     if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]) == hipSuccess) s.total_ms = ms;
-    ctx->pending = false;
     if (host.status != 0) return host.status;
     // every generated node must have been finished by exactly one lane
     if (s.nodes != ctx->expect_nodes) return PPA_ERR_INTERNAL;
     return PPA_OK;
 }
 
-int finish_launch(ppa_ctx *ctx, hipStream_t stream)
+int finish_launch(ppa_ctx *ctx)
 {
-    const int rc = collect_launch(ctx, stream);
+    const int rc = collect_launch(ctx);
     // rows before a failed range's end may be incomplete: no range continues it
     if (rc != PPA_OK) ctx->range_next = 0;
     return rc;
@@ -198,42 +209,7 @@ using namespace ppa;
 
 extern "C" {
 
-const char *ppa_strerror(int code)
-{
-    switch (code) {
-    case PPA_OK: return "ok";
-    case PPA_ERR_INVALID: return "invalid argument";
-    case PPA_ERR_RETRY: return "retry limit exceeded for a slot";
-    case PPA_ERR_NOMEM: return "out of memory";
-    case PPA_ERR_DEVICE: return "HIP runtime error";
-    case PPA_ERR_TIMEOUT: return "kernel watchdog expired";
-    case PPA_ERR_BUDGET: return "request exceeds the memory budget";
-    case PPA_ERR_NOT_FOUND: return "unknown request";
-    case PPA_ERR_CANCELLED: return "request cancelled";
-    case PPA_ERR_AGAIN: return "not ready, try again";
-    case PPA_ERR_INTERNAL: return "internal error";
-    default: return "unknown error";
-    }
-}
-
-uint64_t ppa_p_threshold(double p)
-{
-    if (!(p > 0.0)) return 0;
-    if (p >= 1.0) return 1ull << 32;
-    return (uint64_t)(p * 4294967296.0);  // exact scaling, then floor
-}
-
-int ppa_check_desc(const ppa_graph_desc *desc)
-{
-    if (!desc) return PPA_ERR_INVALID;
-    if (desc->d < 1 || desc->d > PPA_MAX_D) return PPA_ERR_INVALID;
-    if (desc->n < (uint64_t)desc->d + 1 || desc->n > PPA_MAX_N) return PPA_ERR_INVALID;
-    if (desc->p_threshold > (1ull << 32)) return PPA_ERR_INVALID;
-    if (desc->model != PPA_MODEL_COPY && desc->model != PPA_MODEL_BA) return PPA_ERR_INVALID;
-    return PPA_OK;
-}
-
-int ppa_ctx_create(ppa_ctx **out, int device)
+int ppa_ctx_create(ppa_ctx **out, int device, uint64_t scratch_budget_bytes)
 {
     if (!out) return PPA_ERR_INVALID;
     *out = nullptr;
@@ -245,6 +221,7 @@ int ppa_ctx_create(ppa_ctx **out, int device)
     ppa_ctx *c = new (std::nothrow) ppa_ctx;
     if (!c) return PPA_ERR_NOMEM;
     c->device = device;
+    c->scratch_budget = scratch_budget_bytes;
     if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
         hipDeviceGetAttribute(&c->wall_clock_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess) {
         delete c;
@@ -275,6 +252,31 @@ int ppa_ctx_destroy(ppa_ctx *ctx)
     if (ctx->degree_scratch) (void)hipFree(ctx->degree_scratch);
     if (ctx->work) (void)hipFree(ctx->work);
     delete ctx;
+    return PPA_OK;
+}
+
+int ppa_ctx_trim(ppa_ctx *ctx)
+{
+    if (!ctx) return PPA_ERR_INVALID;
+    if (ctx->pending) return PPA_ERR_AGAIN;
+    DeviceGuard g(ctx->device);
+    if (!g.ok()) return PPA_ERR_DEVICE;
+    if (!ctx->work && !ctx->degree_scratch) return PPA_OK;
+    PPA_HIP(hipDeviceSynchronize());  // earlier calls' kernels may still read the work area
+    if (ctx->work) PPA_HIP(hipFree(ctx->work));
+    ctx->work = nullptr;
+    ctx->work_bytes = 0;
+    if (ctx->degree_scratch) PPA_HIP(hipFree(ctx->degree_scratch));
+    ctx->degree_scratch = nullptr;
+    ctx->degree_scratch_len = 0;
+    return PPA_OK;
+}
+
+int ppa_ctx_memory(ppa_ctx *ctx, uint64_t *held, uint64_t *budget)
+{
+    if (!ctx) return PPA_ERR_INVALID;
+    if (held) *held = held_bytes(ctx);
+    if (budget) *budget = ctx->scratch_budget;
     return PPA_OK;
 }
 
@@ -369,7 +371,8 @@ int ppa_ctx_sync(ppa_ctx *ctx, void *stream)
     if (!ctx->pending) return PPA_OK;
     DeviceGuard g(ctx->device);
     if (!g.ok()) return PPA_ERR_DEVICE;
-    return finish_launch(ctx, (hipStream_t)stream);
+    (void)stream;  // the launch's own end event is waited for
+    return finish_launch(ctx);
 }
 
 int ppa_generate(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets, int64_t *d_row_ptr,
@@ -395,6 +398,9 @@ int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t
     uint32_t *deg = d_degree;
     if (!deg) {
         if (ctx->degree_scratch_len < desc->n) {
+            if (ctx->scratch_budget && ctx->work_bytes + desc->n * sizeof(uint32_t) > ctx->scratch_budget)
+                return PPA_ERR_BUDGET;
+            PPA_HIP(hipStreamSynchronize(stream));  // an earlier call may still use the old one
             if (ctx->degree_scratch) PPA_HIP(hipFree(ctx->degree_scratch));
             ctx->degree_scratch = nullptr;
             ctx->degree_scratch_len = 0;

@@ -14,6 +14,7 @@
 // measures launch shapes on the caller's own graphs and keeps the fastest per
 // workload signature.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -30,6 +31,14 @@ struct ppa_batch {
     hipGraphExec_t exec = nullptr;
     uint64_t expect_nodes = 0;
     ppa::LaunchShape shape;
+    // Per-graph completion: the kernels raise flags[i] (mapped, coherent host
+    // memory) once graph i is complete and published; ppa_batch_sync watches
+    // them and keeps, per graph, the host time from ppa_batch_launch to the
+    // first look that found the flag raised.
+    uint32_t *flags = nullptr;
+    uint32_t *d_flags = nullptr;
+    std::vector<double> done_ms;
+    std::chrono::steady_clock::time_point launched;
 };
 
 namespace ppa {
@@ -198,6 +207,7 @@ void destroy(ppa_batch *b)
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->d_tables) (void)hipFree(b->d_tables);
+    if (b->flags) (void)hipHostFree(b->flags);
     delete b;
 }
 
@@ -249,6 +259,14 @@ int ppa_batch_create(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, 
         destroy(b);
         return PPA_ERR_DEVICE;
     }
+    if (hipHostMalloc(&b->flags, count * sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostGetDevicePointer((void **)&b->d_flags, b->flags, 0) != hipSuccess) {
+        destroy(b);
+        return PPA_ERR_NOMEM;
+    }
+    std::memset(b->flags, 0, count * sizeof(uint32_t));
+    b->done_ms.assign(count, -1.0);
     // Capture reset + fill + generate once; launches replay the graph.
     hipStream_t cap = nullptr;
     if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
@@ -257,7 +275,7 @@ int ppa_batch_create(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, 
     }
     bool ok = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
-        ok = enqueue_batch(b->d_tables, tables, b->shape, ctx->scratch, cap) == hipSuccess;
+        ok = enqueue_batch(b->d_tables, tables, b->shape, ctx->scratch, cap, b->d_flags) == hipSuccess;
         hipGraph_t graph = nullptr;
         const bool ended = hipStreamEndCapture(cap, &graph) == hipSuccess;
         ok = ok && ended;
@@ -276,12 +294,19 @@ int ppa_batch_launch(ppa_batch *b, void *stream_)
 {
     if (!b) return PPA_ERR_INVALID;
     ppa_ctx *ctx = b->ctx;
+    // the captured graph resets the context's scratch: never under a pending launch
+    if (ctx->pending) return PPA_ERR_AGAIN;
     DeviceGuard g(ctx->device);
     if (!g.ok()) return PPA_ERR_DEVICE;
     hipStream_t stream = (hipStream_t)stream_;
     ctx->last = ppa_stats{};
     ctx->last.grid_blocks = b->shape.blocks;
     ctx->last.block_threads = b->shape.threads;
+    // no launch of this batch is in flight (the context is not pending), so
+    // its flags are free to clear
+    std::memset(b->flags, 0, b->count * sizeof(uint32_t));
+    std::fill(b->done_ms.begin(), b->done_ms.end(), -1.0);
+    b->launched = std::chrono::steady_clock::now();
     PPA_HIP(hipEventRecord(ctx->ev[0], stream));
     PPA_HIP(hipEventRecord(ctx->ev[1], stream));
     PPA_HIP(hipGraphLaunch(b->exec, stream));
@@ -294,7 +319,50 @@ int ppa_batch_launch(ppa_batch *b, void *stream_)
 int ppa_batch_sync(ppa_batch *b, void *stream)
 {
     if (!b) return PPA_ERR_INVALID;
-    return ppa_ctx_sync(b->ctx, stream);
+    ppa_ctx *ctx = b->ctx;
+    if (!ctx->pending) return ppa_ctx_sync(ctx, stream);
+    double end_ms = -1.0;
+    {
+        DeviceGuard g(ctx->device);
+        if (!g.ok()) return PPA_ERR_DEVICE;
+        // Watch the per-graph flags until the launch's end event completes.
+        auto since = [&]() {
+            return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - b->launched).count();
+        };
+        uint64_t seen = 0, scan = 0;
+        for (;;) {
+            const bool ended = hipEventQuery(ctx->ev[2]) != hipErrorNotReady;
+            if (ended) end_ms = since();
+            double now = -1.0;
+            for (; scan < b->count && b->done_ms[scan] >= 0.0; ++scan) {}
+            for (uint64_t i = scan; i < b->count && seen < b->count; ++i) {
+                if (b->done_ms[i] >= 0.0 || __atomic_load_n(&b->flags[i], __ATOMIC_ACQUIRE) == 0) continue;
+                if (now < 0.0) now = since();
+                b->done_ms[i] = now;
+                ++seen;
+            }
+            if (ended || seen == b->count) break;
+        }
+    }
+    const int rc = ppa_ctx_sync(ctx, stream);
+    // A graph of the seed clique alone has no generated node to raise its
+    // flag; after a launch that succeeded, it and any graph not seen before
+    // the end are complete at the end.
+    if (rc == PPA_OK) {
+        if (end_ms < 0.0)
+            end_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - b->launched).count();
+        for (double &x : b->done_ms)
+            if (x < 0.0) x = end_ms;
+    }
+    return rc;
+}
+
+int ppa_batch_completion_ms(ppa_batch *b, double *done_ms, uint64_t count)
+{
+    if (!b || !done_ms || count != b->count) return PPA_ERR_INVALID;
+    if (b->ctx->pending) return PPA_ERR_AGAIN;
+    for (uint64_t i = 0; i < count; ++i) done_ms[i] = b->done_ms[i];
+    return PPA_OK;
 }
 
 int ppa_batch_destroy(ppa_batch *b)
@@ -321,6 +389,7 @@ int ppa_autotune(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, int3
                  int64_t *d_row_ptr, void *stream, ppa_tuning *best_out)
 {
     if (!ctx || !descs || !count || !d_targets) return PPA_ERR_INVALID;
+    if (ctx->pending) return PPA_ERR_AGAIN;  // its measurements would reset the pending launch's scratch
     // workload signature: n bucket, d, p bucket (single graph) or batch shape
     std::string sig;
     if (count == 1) {

@@ -14,6 +14,43 @@
 
 extern "C" {
 
+// Host-only utilities of the ABI (no HIP), kept here with the serial path so
+// that this file builds with a plain host compiler (the sanitizer build).
+const char *ppa_strerror(int code)
+{
+    switch (code) {
+    case PPA_OK: return "ok";
+    case PPA_ERR_INVALID: return "invalid argument";
+    case PPA_ERR_RETRY: return "retry limit exceeded for a slot";
+    case PPA_ERR_NOMEM: return "out of memory";
+    case PPA_ERR_DEVICE: return "HIP runtime error";
+    case PPA_ERR_TIMEOUT: return "kernel watchdog expired";
+    case PPA_ERR_BUDGET: return "request exceeds the memory budget";
+    case PPA_ERR_NOT_FOUND: return "unknown request";
+    case PPA_ERR_CANCELLED: return "request cancelled";
+    case PPA_ERR_AGAIN: return "not ready, try again";
+    case PPA_ERR_INTERNAL: return "internal error";
+    default: return "unknown error";
+    }
+}
+
+uint64_t ppa_p_threshold(double p)
+{
+    if (!(p > 0.0)) return 0;
+    if (p >= 1.0) return 1ull << 32;
+    return (uint64_t)(p * 4294967296.0);  // exact scaling, then floor
+}
+
+int ppa_check_desc(const ppa_graph_desc *desc)
+{
+    if (!desc) return PPA_ERR_INVALID;
+    if (desc->d < 1 || desc->d > PPA_MAX_D) return PPA_ERR_INVALID;
+    if (desc->n < (uint64_t)desc->d + 1 || desc->n > PPA_MAX_N) return PPA_ERR_INVALID;
+    if (desc->p_threshold > (1ull << 32)) return PPA_ERR_INVALID;
+    if (desc->model != PPA_MODEL_COPY && desc->model != PPA_MODEL_BA) return PPA_ERR_INVALID;
+    return PPA_OK;
+}
+
 int ppa_cpu_generate(const ppa_graph_desc *desc, int32_t *targets, int64_t *row_ptr, uint64_t *retries_out)
 {
     int rc = ppa_check_desc(desc);

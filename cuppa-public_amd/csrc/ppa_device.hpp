@@ -47,9 +47,8 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
     return v;
 }
 
-__device__ __forceinline__ uint64_t wave_sum64(uint32_t v)
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t x)
 {
-    uint64_t x = v;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const uint32_t lo = __shfl_xor((uint32_t)x, off, kWave);

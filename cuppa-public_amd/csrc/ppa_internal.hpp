@@ -21,6 +21,7 @@ struct ppa_ctx {
     uint64_t degree_scratch_len = 0;
     void *work = nullptr;                   // device work area of the partition passes (degree, in-edge CSR)
     size_t work_bytes = 0;
+    uint64_t scratch_budget = 0;            // cap on work + degree_scratch bytes (0 = none)
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     ppa_stats last{};
     bool pending = false;        // a launch awaits ppa_ctx_sync / ppa_batch_sync
@@ -95,8 +96,11 @@ hipError_t enqueue_single(const ppa_ctx *ctx, const ppa_graph_desc &desc, int32_
 uint64_t range_start(const ppa_ctx *ctx, const ppa_graph_desc &desc, uint64_t first, uint64_t last);
 
 // Grows the context's work area to `need` bytes (waiting for `stream` before
-// freeing the old one); PPA_ERR_NOMEM if the device cannot provide it.
+// freeing the old one); PPA_ERR_BUDGET if that would hold more than the
+// context's scratch budget, PPA_ERR_NOMEM if the device cannot provide it.
 int ensure_work(ppa_ctx *ctx, size_t need, hipStream_t stream);
+// Bytes of work memory the context holds (work area + degree scratch).
+uint64_t held_bytes(const ppa_ctx *ctx);
 
 // Host image of a batch's device tables, packed as
 // BatchGraph[count] | slot_base[count+1] | row_base[count+1] | seg_base[nseg+1] |
@@ -124,8 +128,9 @@ hipError_t enqueue_batch(void *d_tables, const BatchTables &t, const LaunchShape
 // A descriptor -> the kernel's graph record (G, row_ptr and offsets set by the caller).
 // prefix_cap: the tuning's prefix_nodes (see prefix_start).
 BatchGraph graph_record(const ppa_graph_desc &desc, uint32_t prefix_cap);
-// Waits for the stream and turns the device scratch into ctx->last.
-int finish_launch(ppa_ctx *ctx, hipStream_t stream);
+// Waits for the pending launch (its end event ctx->ev[2]) and turns the
+// device scratch into ctx->last; clears ctx->pending on every path.
+int finish_launch(ppa_ctx *ctx);
 bool tuning_valid(const ppa_tuning &t);
 
 }  // namespace ppa

@@ -293,8 +293,10 @@ void generate_kernel(GenArgs a)
 
     // Statistics: wavefront-uniform counters fed by ballots; per-lane only
     // where the amount depends on the lane's out-degree.
-    uint32_t s_nodes = 0, s_retries = 0, s_claims = 0, s_sleeps = 0, s_staged = 0;
-    uint32_t c_draws = 0, c_slots = 0, c_pmiss = 0, c_polls = 0, c_stalls = 0;
+    // (64-bit: one wavefront of a small grid may finish 2^30 nodes and more slots)
+    uint64_t s_nodes = 0, s_retries = 0, s_claims = 0, s_sleeps = 0, s_staged = 0;
+    uint64_t c_draws = 0, c_slots = 0;
+    uint32_t c_pmiss = 0, c_polls = 0, c_stalls = 0;
     uint32_t iter = 0, backoff = 1;
     int fail = 0;
 
@@ -619,8 +621,8 @@ void generate_kernel(GenArgs a)
                 if (BATCH) c_slots += dd;
             }
         }
-        s_retries += (uint32_t)__popcll(__ballot(retried));
-        s_nodes += (uint32_t)__popcll(__ballot(finished));
+        s_retries += (uint64_t)__popcll(__ballot(retried));
+        s_nodes += (uint64_t)__popcll(__ballot(finished));
 
         // ---- publish finished rows: coalesced through LDS (wavefront-uniform) ----
         // Each pass publishes the finished rows within 64 nodes of the smallest
@@ -644,7 +646,7 @@ void generate_kernel(GenArgs a)
                 for (int kk = 0; kk < DMAX; ++kk)
                     if (!BATCH || (uint32_t)kk < dd) stage[row * Ld + kk] = val[kk];
             }
-            s_staged += (uint32_t)__popcll(rmask);
+            s_staged += (uint64_t)__popcll(rmask);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -752,8 +754,8 @@ void generate_kernel(GenArgs a)
     }
 
     // ---- statistics: one atomic per counter per wavefront ----
-    const uint32_t draws = wave_sum(c_draws);
-    const uint32_t slots = BATCH ? wave_sum(c_slots) : s_nodes * (uint32_t)DMAX;
+    const uint64_t draws = wave_sum64(c_draws);
+    const uint64_t slots = BATCH ? wave_sum64(c_slots) : s_nodes * (uint64_t)DMAX;
     const uint64_t pmiss = wave_sum64(c_pmiss), polls = wave_sum64(c_polls), stalls = wave_sum64(c_stalls);
     if (lane == 0) {
         unsigned long long *st = a.scratch->stats;

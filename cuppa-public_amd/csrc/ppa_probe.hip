@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ppa_model.hpp"
+
 namespace {
 
 __device__ __forceinline__ uint64_t mix64(uint64_t x)
@@ -73,6 +75,53 @@ __global__ __launch_bounds__(256) void gather_lds_kernel(const int32_t *table, u
     sink[tid] = acc;
 }
 
+// gather_lds_kernel with the load's cache-policy bits spelled out (gfx950:
+// sc0 = 1, nt = 2, sc1 = 16): which form of an isolated 4-byte read reaches
+// memory as the smallest request, and at what rate.
+template <int AUX>
+__global__ __launch_bounds__(256) void gather_policy_kernel(const int32_t *table, uint64_t table_len, uint32_t iters,
+                                                            uint64_t seed, int32_t *sink)
+{
+    __shared__ int32_t rows[4][8 * 64];
+    int32_t *const mine = rows[threadIdx.x >> 6];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t acc = 0;
+#pragma unroll 1
+    for (uint32_t i = 0; i < iters; i += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint64_t pos = __umul64hi(mix64(seed ^ (tid << 20) ^ (uint64_t)(i + u)), table_len);
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(table + pos),
+                                             (__attribute__((address_space(3))) void *)(mine + u * 64), 4, 0, AUX);
+        }
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += mine[u * 64 + lane];
+        __builtin_amdgcn_wave_barrier();
+    }
+    sink[tid] = acc;
+}
+
+// The generator's draw alone: `iters` keyed Philox-4x32-10 attempts per lane
+// (the coin and the mulhi64 pair mapping included), no memory traffic but
+// one store per lane. Its rate is the integer-VALU roof of the draws.
+__global__ __launch_bounds__(256) void philox_kernel(uint32_t iters, uint64_t seed, int32_t *sink)
+{
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    uint64_t acc = 0;
+    const uint64_t t = (1ull << 29) + tid;
+#pragma unroll 1
+    for (uint32_t i = 0; i < iters; ++i) {
+        const ppa::Attempt a = ppa::attempt(k0, k1, 858993459ull, t, i & 7u, i >> 3, t * 8u);
+        acc += a.pair + (a.direct ? 1u : 0u);
+    }
+    sink[tid] = (int32_t)(acc ^ (acc >> 32));
+}
+
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
 
 // gather_lds_kernel<true> plus, per batch, the row publication the generator
@@ -121,14 +170,16 @@ extern "C" {
 // mode: 0 plain loads, 1 non-temporal, 2 into LDS, 3 into LDS non-temporal,
 // 4 as 3 with one 2 KB block of agent-scope row stores per wavefront and
 // batch (written into the table's first 2^28 ints: table_len >= 2^28): the
-// generator's mix of copy reads and row publication.
+// generator's mix of copy reads and row publication; 5..10 into LDS with the
+// cache-policy bits sc0, sc1, sc0 sc1, nt sc0, nt sc1, nt sc0 sc1; 11 the
+// Philox draws alone (`iters` per lane; table and table_len unused).
 // Returns 0, or -1 for bad arguments, or -4 for a HIP error.
 int ppa_probe_gather(const int32_t *table, uint64_t table_len, uint32_t blocks, uint32_t threads, uint32_t iters,
                      int mode, int32_t *sink, void *stream, float *ms_out)
 {
     if (!table || !sink || !ms_out || table_len == 0 || blocks == 0 || threads == 0 || threads > 256 ||
-        threads % 64 != 0 || iters == 0 || iters % 8 != 0 || mode < 0 || mode > 4 ||
-        (mode >= 4 && table_len < (1ull << 28)))
+        threads % 64 != 0 || iters == 0 || iters % 8 != 0 || mode < 0 || mode > 11 ||
+        (mode == 4 && table_len < (1ull << 28)))
         return -1;
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -145,9 +196,29 @@ int ppa_probe_gather(const int32_t *table, uint64_t table_len, uint32_t blocks, 
     else if (mode == 3)
         hipLaunchKernelGGL(gather_lds_kernel<true>, dim3(blocks), dim3(threads), 0, s, table, table_len, iters, seed,
                            sink);
-    else
+    else if (mode == 4)
         hipLaunchKernelGGL(gather_store_kernel, dim3(blocks), dim3(threads), 0, s, const_cast<int32_t *>(table),
                            table_len, 1ull << 28, iters, seed, sink);
+    else if (mode == 5)
+        hipLaunchKernelGGL(gather_policy_kernel<1>, dim3(blocks), dim3(threads), 0, s, table, table_len, iters, seed,
+                           sink);
+    else if (mode == 6)
+        hipLaunchKernelGGL(gather_policy_kernel<16>, dim3(blocks), dim3(threads), 0, s, table, table_len, iters, seed,
+                           sink);
+    else if (mode == 7)
+        hipLaunchKernelGGL(gather_policy_kernel<17>, dim3(blocks), dim3(threads), 0, s, table, table_len, iters, seed,
+                           sink);
+    else if (mode == 8)
+        hipLaunchKernelGGL(gather_policy_kernel<3>, dim3(blocks), dim3(threads), 0, s, table, table_len, iters, seed,
+                           sink);
+    else if (mode == 9)
+        hipLaunchKernelGGL(gather_policy_kernel<18>, dim3(blocks), dim3(threads), 0, s, table, table_len, iters, seed,
+                           sink);
+    else if (mode == 10)
+        hipLaunchKernelGGL(gather_policy_kernel<19>, dim3(blocks), dim3(threads), 0, s, table, table_len, iters, seed,
+                           sink);
+    else
+        hipLaunchKernelGGL(philox_kernel, dim3(blocks), dim3(threads), 0, s, iters, seed, sink);
     if (err == hipSuccess) err = hipGetLastError();
     if (err == hipSuccess) err = hipEventRecord(e1, s);
     if (err == hipSuccess) err = hipEventSynchronize(e1);

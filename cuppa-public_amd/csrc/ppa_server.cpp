@@ -494,7 +494,7 @@ int ppa_server_replay(ppa_server *s, const ppa_request *reqs, const double *arri
                     latency_ms[i] = r.done_ms - (base + arrival_ms[i]);
                     if (wait_ms) wait_ms[i] = r.start_ms - (base + arrival_ms[i]);
                     last = std::max(last, r.done_ms - base);
-                    ppa_server_release(s, id);
+                    if (!s->cfg.keep_results) ppa_server_release(s, id);
                     ++done;
                 } else {
                     still.push_back(id);
@@ -502,6 +502,10 @@ int ppa_server_replay(ppa_server *s, const ppa_request *reqs, const double *arri
             }
             live.swap(still);
         }
+        // Kept results never free memory: a queued request that does not fit
+        // while nothing runs would wait forever.
+        if (s->cfg.keep_results && s->running == 0 && s->ready.empty() && !s->queue.empty())
+            return PPA_ERR_BUDGET;
     }
     if (makespan_ms) *makespan_ms = last;
     return PPA_OK;

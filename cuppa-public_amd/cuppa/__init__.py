@@ -73,7 +73,7 @@ class Stats(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_uint64), ("streams", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
                 ("batch_nodes", ctypes.c_uint64), ("solo_nodes", ctypes.c_uint64), ("plain", ctypes.c_uint32),
-                ("solo_grid_div", ctypes.c_uint32)]
+                ("solo_grid_div", ctypes.c_uint32), ("keep_results", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class Request(ctypes.Structure):
@@ -98,8 +98,10 @@ _SIGNATURES = [
     ("ppa_strerror", ctypes.c_char_p, [ctypes.c_int]),
     ("ppa_p_threshold", ctypes.c_uint64, [ctypes.c_double]),
     ("ppa_check_desc", ctypes.c_int, [ctypes.POINTER(GraphDesc)]),
-    ("ppa_ctx_create", ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int]),
+    ("ppa_ctx_create", ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int, ctypes.c_uint64]),
     ("ppa_ctx_destroy", ctypes.c_int, [_VP]),
+    ("ppa_ctx_trim", ctypes.c_int, [_VP]),
+    ("ppa_ctx_memory", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     ("ppa_ctx_set_tuning", ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
     ("ppa_ctx_get_tuning", ctypes.c_int, [_VP, ctypes.POINTER(Tuning)]),
     ("ppa_get_stats", ctypes.c_int, [_VP, ctypes.POINTER(Stats)]),
@@ -117,6 +119,7 @@ _SIGNATURES = [
      [_VP, ctypes.POINTER(GraphDesc), ctypes.c_uint64, _VP, _VP, ctypes.POINTER(_VP)]),
     ("ppa_batch_launch", ctypes.c_int, [_VP, _VP]),
     ("ppa_batch_sync", ctypes.c_int, [_VP, _VP]),
+    ("ppa_batch_completion_ms", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_double), ctypes.c_uint64]),
     ("ppa_batch_destroy", ctypes.c_int, [_VP]),
     ("ppa_generate_batch", ctypes.c_int, [_VP, ctypes.POINTER(GraphDesc), ctypes.c_uint64, _VP, _VP, _VP]),
     ("ppa_autotune", ctypes.c_int,
@@ -205,12 +208,23 @@ class Graph:
 
 
 class Context:
-    """One device context (ppa_ctx)."""
+    """One device context (ppa_ctx). scratch_budget caps the work memory of
+    the degree pass and the CSR forms (0 = no cap)."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, scratch_budget: int = 0):
         self._h = ctypes.c_void_p()
         self.device = device
-        _check(lib().ppa_ctx_create(ctypes.byref(self._h), device), "ppa_ctx_create")
+        _check(lib().ppa_ctx_create(ctypes.byref(self._h), device, scratch_budget), "ppa_ctx_create")
+
+    def trim(self):
+        """Frees the work memory the context keeps between calls (ppa_ctx_trim)."""
+        _check(lib().ppa_ctx_trim(self._h), "ppa_ctx_trim")
+
+    def memory(self) -> dict:
+        """{'held': bytes of work memory held, 'budget': its cap (0 = none)}."""
+        held, budget = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().ppa_ctx_memory(self._h, ctypes.byref(held), ctypes.byref(budget)), "ppa_ctx_memory")
+        return {"held": held.value, "budget": budget.value}
 
     def close(self):
         if self._h:
@@ -376,10 +390,19 @@ class Batch:
         self.launch(stream)
         self.sync(stream)
 
+    def completion_ms(self):
+        """Per-graph ms from the last launch to its graph's completion
+        (ppa_batch_completion_ms; -1 for a graph that never completed)."""
+        out = (ctypes.c_double * len(self.descs))()
+        _check(lib().ppa_batch_completion_ms(self._h, out, len(self.descs)), "ppa_batch_completion_ms")
+        return list(out)
+
     def close(self):
         if self._h:
             lib().ppa_batch_destroy(self._h)
             self._h = ctypes.c_void_p()
+        # the output buffers were kept alive only for the native batch
+        self.targets = self.row_ptr = None
 
     def __del__(self):
         try:
@@ -400,11 +423,12 @@ class Server:
     """Continuous-batching request loop (ppa_server)."""
 
     def __init__(self, ctx: "Context", budget_bytes: int, streams: int = 0, max_batch: int = 0,
-                 batch_nodes: int = 0, solo_nodes: int = 0, plain: bool = False, solo_grid_div: int = 0):
+                 batch_nodes: int = 0, solo_nodes: int = 0, plain: bool = False, solo_grid_div: int = 0,
+                 keep_results: bool = False):
         self.ctx = ctx
         self._h = ctypes.c_void_p()
         cfg = ServerConfig(budget_bytes, streams, max_batch, batch_nodes, solo_nodes, 1 if plain else 0,
-                           solo_grid_div)
+                           solo_grid_div, 1 if keep_results else 0, 0)
         _check(lib().ppa_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(self._h)), "ppa_server_create")
 
     def submit(self, rid: int, n: int, d: int, p_thresh: int, seed: int, model: int = MODEL_COPY) -> int:

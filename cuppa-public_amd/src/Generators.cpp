@@ -229,14 +229,67 @@ int cuPPA::generateGraph(const ExperimentConfiguration &conf)
     return run(conf, "StepwiseTiming-Original.csv", false);
 }
 
-// cuPPA_2.cu:181-198 doubles the launch width while its queues stay below
-// half capacity. One persistent launch has no stages, and every launch shape
-// yields the same graph, so the online form here tunes on the graph itself:
-// the persistent grid at 1/8, 1/4, 1/2 and all of its size, stopping when a
-// doubling gains < 3 %, one StepwiseTiming row per trial.
+// cuPPA_2.cu:181-198 generates one graph stage by stage and, after each
+// stage whose queues stayed below half capacity, doubles both its thread
+// count and the next stage's size (up to the configured maxima). Here the
+// stages are consecutive node ranges of one graph (ppa_generate_range), each
+// a persistent launch: the first runs on 1/8 of the full grid, and after each
+// range whose time per generated node beat the previous range's by 3 % or
+// more, the grid and the next range double (up to the full grid and n/8
+// nodes). Every launch shape gives the same graph, so the tuning changes only
+// the time. One StepwiseTiming row per range (cuPPA_2.cu:204-211);
+// `Computation Time` is the whole graph's wall time.
 int cuPPA::generateGraphAdaptive(const ExperimentConfiguration &conf)
 {
     return run(conf, "StepwiseTiming-Adaptive.csv", true);
+}
+
+int cuPPA::generateAdaptiveRanges(const ppa_graph_desc &g, const ppa_tuning &tu, std::vector<ppa_stats> *rows,
+                                  std::vector<uint64_t> *range_nodes, double *wall_ms)
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 1;
+    // the library's single-call grid: 16 wavefronts per CU
+    const uint32_t threads = tu.block_threads ? tu.block_threads : 256;
+    const uint64_t full = (uint64_t)cus * std::max<uint32_t>(1, 1024 / threads);
+    uint64_t grid = std::max<uint64_t>(1, full / 8);
+    const uint64_t cap = std::max<uint64_t>(1ull << 16, g.n / 8);
+    uint64_t size = std::min<uint64_t>(cap, std::max<uint64_t>(1ull << 16, g.n / 64));
+    bool final_tuning = false;
+    double prev = -1.0;
+    uint64_t u = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = PPA_OK;
+    while (u < g.n) {
+        uint64_t last = std::min<uint64_t>(g.n, u + size);
+        if (g.n - last < size / 2) last = g.n;  // no short tail range
+        ppa_tuning t = tu;
+        t.max_blocks = (uint32_t)grid;
+        if ((rc = ppa_ctx_set_tuning(_ctx, &t)) != PPA_OK) break;
+        if ((rc = ppa_generate_range(_ctx, &g, _dGraph, _dRowPtr, u, last, nullptr)) != PPA_OK) break;
+        if ((rc = ppa_ctx_sync(_ctx, nullptr)) != PPA_OK) break;
+        ppa_stats s{};
+        ppa_get_stats(_ctx, &s);
+        rows->push_back(s);
+        range_nodes->push_back(last - u);
+        const uint64_t lo = std::max<uint64_t>(u, (uint64_t)g.d + 1);
+        const double per_node = s.kernel_ms / (double)std::max<uint64_t>(1, last > lo ? last - lo : 0);
+        if (!final_tuning && (prev < 0.0 || per_node < 0.97 * prev)) {
+            grid = std::min<uint64_t>(full, grid * 2);
+            size = std::min<uint64_t>(cap, size * 2);
+            if (grid == full && size == cap) final_tuning = true;
+        }
+        prev = per_node;
+        u = last;
+    }
+    *wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    // later calls on this context keep the grid the tuning reached
+    ppa_tuning kept = tu;
+    kept.max_blocks = grid >= full ? 0u : (uint32_t)grid;
+    const int r2 = ppa_ctx_set_tuning(_ctx, &kept);
+    return rc ? rc : r2;
 }
 
 int cuPPA::run(const ExperimentConfiguration &conf, const char *timingCsv, bool adaptive)
@@ -244,7 +297,7 @@ int cuPPA::run(const ExperimentConfiguration &conf, const char *timingCsv, bool 
     const ppa_graph_desc g = desc(conf);
     int rc = ppa_check_desc(&g);
     if (rc) return rc;
-    if (!_ctx && (rc = ppa_ctx_create(&_ctx, conf._device)) != PPA_OK) return rc;
+    if (!_ctx && (rc = ppa_ctx_create(&_ctx, conf._device, 0)) != PPA_OK) return rc;
     ppa_tuning tu{};
     if (conf._threadsGiven) {
         const int threads = conf._threadsPerBlock < 64 ? 64 : conf._threadsPerBlock;
@@ -262,59 +315,51 @@ int cuPPA::run(const ExperimentConfiguration &conf, const char *timingCsv, bool 
     std::printf("--------------------------------------------------------------------------------\n");
     std::printf("Graph: %.2f MB\t| RowPtr: %.2f MB\n", g.n * g.d * 4.0 / 1048576.0, (g.n + 1) * 8.0 / 1048576.0);
     // with --csr-file (and one launch shape) the file is written while the graph is generated
-    const bool streamCsr = !conf._csrFile.empty() && !adaptive;
+    const bool ranged = adaptive && !conf._blocksGiven;
+    const bool streamCsr = !conf._csrFile.empty() && !ranged;
     ppa_stats st{};
-    if (streamCsr) {
-        rc = generateStreamingCsr(conf, g, &st);
+    std::vector<ppa_stats> stages;        // one StepwiseTiming row each
+    std::vector<uint64_t> stage_nodes;
+    double computation_ms = 0.0;
+    if (ranged) {
+        rc = generateAdaptiveRanges(g, tu, &stages, &stage_nodes, &computation_ms);
+        for (const ppa_stats &s : stages) {
+            st.draws += s.draws;
+            st.retries += s.retries;
+            st.prefetch_misses += s.prefetch_misses;
+            st.grid_blocks = s.grid_blocks;  // the grid the tuning ended at
+            st.block_threads = s.block_threads;
+        }
     } else {
-        rc = ppa_generate(_ctx, &g, _dGraph, _dRowPtr, nullptr);
-        ppa_get_stats(_ctx, &st);
+        if (streamCsr) {
+            rc = generateStreamingCsr(conf, g, &st);
+        } else {
+            rc = ppa_generate(_ctx, &g, _dGraph, _dRowPtr, nullptr);
+            ppa_get_stats(_ctx, &st);
+        }
+        stages.push_back(st);
+        stage_nodes.push_back(g.n);
+        computation_ms = st.total_ms;
     }
     if (rc) return rc;
-    std::vector<ppa_stats> trials{st};
-    if (adaptive && !conf._blocksGiven) {
-        // the first run's grid is the full one; try 1/8, 1/4, 1/2 of it upwards
-        const uint64_t full = st.grid_blocks;
-        ppa_stats best = st;
-        double prev = -1;
-        for (int shift = 3; shift >= 1; --shift) {
-            const uint64_t blocks = full >> shift;
-            if (blocks == 0) continue;
-            ppa_tuning t2 = tu;
-            t2.max_blocks = (uint32_t)blocks;
-            if ((rc = ppa_ctx_set_tuning(_ctx, &t2)) != PPA_OK) return rc;
-            if ((rc = ppa_generate(_ctx, &g, _dGraph, _dRowPtr, nullptr)) != PPA_OK) return rc;
-            ppa_stats s2{};
-            ppa_get_stats(_ctx, &s2);
-            trials.push_back(s2);
-            if (s2.kernel_ms < best.kernel_ms) best = s2;
-            if (prev > 0 && s2.kernel_ms > 0.97 * prev) break;  // doubling no longer pays
-            prev = s2.kernel_ms;
-        }
-        // the kept shape for later calls on this context
-        ppa_tuning t3 = tu;
-        t3.max_blocks = best.grid_blocks == full ? 0u : (uint32_t)best.grid_blocks;
-        if ((rc = ppa_ctx_set_tuning(_ctx, &t3)) != PPA_OK) return rc;
-        st = best;
-    }
     std::printf("BLOCKS: %llu\t| THREADS PER BLOCK: %llu\t| DRAWS: %llu\t| RETRIES: %llu\t| PARKED: %llu\n",
                 (unsigned long long)st.grid_blocks, (unsigned long long)st.block_threads,
                 (unsigned long long)st.draws, (unsigned long long)st.retries,
                 (unsigned long long)st.prefetch_misses);
     {
-        // cuPPA_1.cu:84-87,217-224: one row per stage; here one per launch
-        // (the adaptive mode's trials, the plain mode's single launch).
+        // cuPPA_1.cu:84-87,217-224 and cuPPA_2.cu:204-211: one row per stage
+        // (the adaptive mode's node ranges, the plain mode's single launch).
         std::ofstream csv(timingCsv);
         csv << "Round,nBlocks,nThreads,Edges,MaxQ,Initialization Time,Computation Time\n";
-        const size_t rows = std::min<size_t>(trials.size(), conf._N_STAGE_TIMINGS > 0 ? conf._N_STAGE_TIMINGS : 0);
+        const size_t rows = std::min<size_t>(stages.size(), conf._N_STAGE_TIMINGS > 0 ? conf._N_STAGE_TIMINGS : 0);
         for (size_t i = 0; i < rows; ++i) {
-            const ppa_stats &x = trials[i];
-            csv << i + 1 << "," << x.grid_blocks << "," << x.block_threads << "," << g.n * g.d << ","
+            const ppa_stats &x = stages[i];
+            csv << i + 1 << "," << x.grid_blocks << "," << x.block_threads << "," << stage_nodes[i] * g.d << ","
                 << x.prefetch_misses << "," << (x.total_ms - x.kernel_ms) * 1e-3 << "," << x.kernel_ms * 1e-3
                 << "\n";
         }
     }
-    std::printf("Computation Time: %f\n\n\n", st.total_ms * 1e-3);
+    std::printf("Computation Time: %f\n\n\n", computation_ms * 1e-3);
     if (conf._FLAG_SHOW_DEGREE_DISTRIBUTION && (rc = printDegreeDistribution(conf)) != PPA_OK) return rc;
     if ((rc = writeDeviceGraphFiles(conf, _ctx, g, _dGraph, _dRowPtr, streamCsr)) != PPA_OK) return rc;
     std::cout << "Ended" << std::endl;
@@ -405,7 +450,7 @@ int SerialBA::generateGraph(const ExperimentConfiguration &conf)
     if (rc) return rc;
     // SerialBA.cpp:24-37: the graph's size, then the allocation
     log << "Required Memory for Graph: " << (g.n * g.d * 4.0 + (g.n + 1) * 8.0) / (1024.0 * 1024.0) << std::endl;
-    if (!_ctx && (rc = ppa_ctx_create(&_ctx, conf._device)) != PPA_OK) return rc;
+    if (!_ctx && (rc = ppa_ctx_create(&_ctx, conf._device, 0)) != PPA_OK) return rc;
     if (!_dGraph) {
         if (hipMalloc(&_dGraph, g.n * g.d * sizeof(int32_t)) != hipSuccess) {
             log << "Not Enough Memory!" << std::endl;

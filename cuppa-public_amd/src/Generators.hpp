@@ -26,9 +26,11 @@ public:
     // cuPPA_1.cu:9-254. Returns 0 or a PPA_ERR_* code.
     int generateGraph(const ExperimentConfiguration &conf);
     // cuPPA_2.cu:9-241: the reference grows threads and stage size online
-    // while its queues stay half empty; here one persistent launch already
-    // spans every stage, so this runs the same generator and records the
-    // run under the adaptive CSV name.
+    // while its queues stay half empty. Here one graph is generated as
+    // consecutive node ranges (ppa_generate_range) starting on 1/8 of the
+    // persistent grid; the grid and the next range double while the time per
+    // node improves. One StepwiseTiming row per range; Computation Time is
+    // the whole graph's. With -b given, one launch of that grid instead.
     int generateGraphAdaptive(const ExperimentConfiguration &conf);
     // cuPPA.cu:103-165
     int printDegreeDistribution(const ExperimentConfiguration &conf);
@@ -41,6 +43,10 @@ private:
     // as they finish: range k's rows are copied out and written while range
     // k+1 is generated. *total sums the ranges' statistics.
     int generateStreamingCsr(const ExperimentConfiguration &conf, const ppa_graph_desc &g, ppa_stats *total);
+    // The adaptive mode's ranges: their statistics, their node counts and the
+    // wall time of the whole graph.
+    int generateAdaptiveRanges(const ppa_graph_desc &g, const ppa_tuning &tu, std::vector<ppa_stats> *rows,
+                               std::vector<uint64_t> *range_nodes, double *wall_ms);
     ppa_graph_desc desc(const ExperimentConfiguration &conf) const;
 
     int64_t _N;

@@ -143,8 +143,20 @@ uint64_t ppa_p_threshold(double p);
 int ppa_check_desc(const ppa_graph_desc *desc);
 
 /* ---- context (replaces the cuPPA object, cuPPA.cuh:84-105) ---------------- */
-int ppa_ctx_create(ppa_ctx **out, int device);
+/* scratch_budget_bytes caps the device work memory the context may hold for
+ * the degree pass and the in-edge / undirected CSR (0 = no cap). The
+ * generators themselves need only the context's fixed 128-byte scratch.
+ * Work memory is kept between calls (the next call of the same size reuses
+ * it) until ppa_ctx_trim or ppa_ctx_destroy. Over the cap, the degree pass
+ * counts with one atomic per slot instead of its partitioned passes (same
+ * result, slower), and the CSR forms return PPA_ERR_BUDGET. */
+int ppa_ctx_create(ppa_ctx **out, int device, uint64_t scratch_budget_bytes);
 int ppa_ctx_destroy(ppa_ctx *ctx);
+/* Frees the work memory the context holds (waits for the device first).
+ * PPA_ERR_AGAIN while a launch is pending. */
+int ppa_ctx_trim(ppa_ctx *ctx);
+/* Work memory now held and the cap it is held to (0 = none); either nullable. */
+int ppa_ctx_memory(ppa_ctx *ctx, uint64_t *held_bytes, uint64_t *budget_bytes);
 int ppa_ctx_set_tuning(ppa_ctx *ctx, const ppa_tuning *tuning);
 int ppa_ctx_get_tuning(ppa_ctx *ctx, ppa_tuning *out);
 int ppa_get_stats(ppa_ctx *ctx, ppa_stats *out);
@@ -156,10 +168,13 @@ int ppa_get_stats(ppa_ctx *ctx, ppa_stats *out);
  * returns the kernel's status. */
 int ppa_generate(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets,
                  int64_t *d_row_ptr, void *stream);
-/* Same work without the final wait; the status is read by ppa_ctx_sync on the
- * same stream. Until then the launch owns the context's scratch: another
- * ppa_generate*, ppa_degree_histogram, ppa_histogram_of_degrees, ppa_in_csr
- * or ppa_undirected_csr call on the context returns PPA_ERR_AGAIN. */
+/* Same work without the final wait; the status is read by ppa_ctx_sync. Until
+ * then the launch owns the context's scratch: another ppa_generate*,
+ * ppa_batch_launch, ppa_generate_batch, ppa_autotune, ppa_degree_histogram,
+ * ppa_histogram_of_degrees, ppa_in_csr or ppa_undirected_csr call on the
+ * context returns PPA_ERR_AGAIN. ppa_ctx_sync waits for the launch's own end
+ * (an event recorded on the launch's stream), whichever stream it is given,
+ * and frees the context whatever it returns. */
 int ppa_generate_async(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets,
                        int64_t *d_row_ptr, void *stream);
 int ppa_ctx_sync(ppa_ctx *ctx, void *stream);
@@ -232,8 +247,20 @@ typedef struct ppa_batch ppa_batch;
  * hipGraph. The batch borrows the context: one launch in flight per context. */
 int ppa_batch_create(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, int32_t *d_targets,
                      int64_t *d_row_ptr, ppa_batch **out);
-int ppa_batch_launch(ppa_batch *batch, void *stream);   /* replays the graph, does not wait */
-int ppa_batch_sync(ppa_batch *batch, void *stream);     /* waits and returns the kernel status */
+/* Replays the graph, does not wait. PPA_ERR_AGAIN while the context has a
+ * launch pending (ppa_generate_async, ppa_generate_range or another batch). */
+int ppa_batch_launch(ppa_batch *batch, void *stream);
+/* Waits and returns the kernel status. While it waits it watches the graphs'
+ * completion flags (raised by the kernels once a graph is complete and
+ * published, before the rest of the launch ends). */
+int ppa_batch_sync(ppa_batch *batch, void *stream);
+/* After ppa_batch_sync: done_ms[i] = host milliseconds from ppa_batch_launch
+ * to the first look that found graph i complete. Graphs not seen before the
+ * launch ended (a graph of the seed clique alone has nothing to generate) get
+ * its end if it succeeded; after a failed launch they stay -1. count must
+ * equal the batch's graph count. No counterpart in the reference: the
+ * per-graph latency of a batched ensemble. */
+int ppa_batch_completion_ms(ppa_batch *batch, double *done_ms, uint64_t count);
 int ppa_batch_destroy(ppa_batch *batch);
 /* create + launch + sync + destroy */
 int ppa_generate_batch(ppa_ctx *ctx, const ppa_graph_desc *descs, uint64_t count, int32_t *d_targets,
@@ -273,6 +300,11 @@ typedef struct ppa_server_config {
                                single ppa_generate call would (0 = 3; 1 = the same): its
                                chain-bound workgroups hold wave slots that concurrent launches
                                could use; ignored in plain mode */
+    uint32_t keep_results;  /* 1: ppa_server_replay leaves finished requests in place for the
+                               caller to poll and release (their memory stays held, so the
+                               budget must cover every request of the trace); 0: it releases
+                               each request as soon as it finishes */
+    uint32_t reserved;
 } ppa_server_config;
 
 typedef struct ppa_request {
@@ -315,8 +347,10 @@ int ppa_server_usage(ppa_server *srv, uint64_t *used_bytes, uint64_t *budget_byt
                      uint64_t *running);
 /* Replays a trace in real time: request i is submitted when the clock reaches
  * arrival_ms[i] (ms from the start of the call) and released as soon as it
- * finishes; latency_ms[i] = finish - arrival, wait_ms[i] (nullable) = launch
- * start - arrival. makespan_ms = last finish. */
+ * finishes (kept for the caller with keep_results; then PPA_ERR_BUDGET if the
+ * kept requests leave no room for the next one); latency_ms[i] = finish -
+ * arrival, wait_ms[i] (nullable) = launch start - arrival. makespan_ms = last
+ * finish. */
 int ppa_server_replay(ppa_server *srv, const ppa_request *reqs, const double *arrival_ms, uint64_t count,
                       double *latency_ms, double *wait_ms, double *makespan_ms);
 

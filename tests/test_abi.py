@@ -97,7 +97,7 @@ def test_ctx_create_reports_missing_gpu(L):
     if torch.cuda.is_available():
         pytest.skip("a GPU is present")
     h = ctypes.c_void_p()
-    assert L.ppa_ctx_create(ctypes.byref(h), 0) == -4  # PPA_ERR_DEVICE, no crash
+    assert L.ppa_ctx_create(ctypes.byref(h), 0, 0) == -4  # PPA_ERR_DEVICE, no crash
 
 
 @pytest.mark.parametrize("n,d,p,seed", [(6, 5, 0.5, 1), (20_000, 1, 0.5, 2), (50_000, 8, 0.2, 11),

@@ -133,3 +133,50 @@ def test_autotune_single_and_batch(ctx, oracle_mod):
         _check(oracle_mod, specs, bd, bt, br)
     finally:
         ctx.set_tuning(watchdog_ms=20000)
+
+
+def test_batch_launch_refused_while_a_launch_is_pending(ctx, oracle_mod):
+    """A batch launch resets the context's scratch, so it must not start under
+    a pending ppa_generate_async: PPA_ERR_AGAIN, and the pending graph still
+    comes out bit for bit."""
+    import ctypes
+    import torch
+    import cuppa
+    n, d, p, seed = 300_000, 8, 0.2, 21
+    T = oracle_mod.p_threshold(p)
+    desc = cuppa.make_desc(n, d, None, seed, T)
+    targets = torch.empty(n * d, dtype=torch.int32, device="cuda")
+    specs = _specs(oracle_mod, 6, seed=13)
+    descs, bt, br = _alloc(specs)
+    b = cuppa.Batch(ctx, descs, bt, br)
+    try:
+        ctx.generate_into(desc, targets, None, sync=False)
+        assert cuppa.lib().ppa_batch_launch(b._h, cuppa._stream_handle(None)) == -9
+        rc = cuppa.lib().ppa_autotune(ctx.handle, descs, len(descs), ctypes.c_void_p(bt.data_ptr()),
+                                      ctypes.c_void_p(br.data_ptr()), cuppa._stream_handle(None),
+                                      ctypes.byref(cuppa.Tuning()))
+        assert rc == -9
+        ctx.sync()
+        want, _, _ = oracle_mod.generate(n, d, T, seed)
+        assert np.array_equal(targets.cpu().numpy(), want)
+        b.run()  # free again
+        _check(oracle_mod, specs, descs, bt, br)
+    finally:
+        b.close()
+
+
+def test_batch_completion_times(ctx, oracle_mod):
+    """Every graph's completion is observed, no later than the launch's end."""
+    import time
+    import cuppa
+    specs = _specs(oracle_mod, 40, seed=17)
+    descs, targets, row_ptr = _alloc(specs)
+    b = cuppa.Batch(ctx, descs, targets, row_ptr)
+    t0 = time.perf_counter()
+    b.run()
+    wall = (time.perf_counter() - t0) * 1e3
+    done = b.completion_ms()
+    b.close()
+    assert len(done) == len(specs)
+    assert all(0.0 <= x <= wall + 1.0 for x in done), (min(done), max(done), wall)
+    _check(oracle_mod, specs, descs, targets, row_ptr)

@@ -84,20 +84,27 @@ def test_runner_csr_written_while_generating(tmp_path, oracle_mod):
     assert np.array_equal(idx, G)
 
 
-def test_runner_adaptive_trials(tmp_path, oracle_mod):
-    """`-A cuppa-adaptive` tunes the grid on the graph itself: one CSV row per
-    trial launch (the full grid, then 1/8, 1/4, ... of it), and the graph left
-    in the buffer is the oracle's whatever the trial."""
+def test_runner_adaptive_ranges(tmp_path, oracle_mod):
+    """`-A cuppa-adaptive` generates one graph as consecutive node ranges
+    (cuPPA_2.cu:181-211 semantics): the first of 2^16 nodes, each later one
+    twice the previous while the time per node improves, up to n/8; one CSV
+    row per range, whose edges add up to the graph's; the graph is the
+    oracle's."""
     n, d, p, seed = 2_000_000, 8, 0.2, 21
     r = subprocess.run([RUNNER, "-A", "cuppa-adaptive", "-n", str(n), "-x", str(d), "-a", str(p), "--seed",
-                        str(seed), "--show-time-n-stages", "8", "--edges-file", str(tmp_path / "g.edges")],
+                        str(seed), "--show-time-n-stages", "1000", "--edges-file", str(tmp_path / "g.edges")],
                        cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    assert "Computation Time: " in r.stdout
     lines = (tmp_path / "StepwiseTiming-Adaptive.csv").read_text().splitlines()
     rows = [l.split(",") for l in lines[1:]]
-    assert 2 <= len(rows) <= 4
-    assert all(row[3] == str(n * d) for row in rows)
-    assert int(rows[1][1]) < int(rows[0][1])  # the second trial is a fraction of the full grid
+    edges_per_row = [int(row[3]) for row in rows]
+    assert len(rows) >= 3
+    assert sum(edges_per_row) == n * d
+    assert edges_per_row[0] == (1 << 16) * d
+    assert edges_per_row[1] == (1 << 17) * d  # the first range always doubles the next
+    assert max(edges_per_row) <= (n // 8 + n // 16) * d  # ranges stop doubling at n/8 (the tail joins the last)
+    assert all(float(row[6]) > 0 for row in rows)
     G, _, _ = oracle_mod.generate(n, d, oracle_mod.p_threshold(p), seed)
     edges = np.loadtxt(tmp_path / "g.edges", dtype=np.int64)
     assert np.array_equal(edges[d * (d + 1) // 2:, 1], G[(d + 1) * d:])

@@ -10,9 +10,9 @@ Config 2 (n = 2^28, d = 8, p = 0.5, seed = 7):
   * size-independent properties: targets of t in [0, t), distinct within a
     row, row_ptr[t] = t*d, the degree histogram equal to an independent torch
     recount, sum(hist) = n and sum(deg * hist) = d(d+1) + 2(n-d-1)d.
-Config 3 (n = 2^30, d = 8, p = 0.2, seed = 11): the first 2^22 rows plus 64
-Philox-placed ranges of 2^16 rows and the last 2^16 rows through the
-row-consistency verifier.
+Config 3 (n = 2^30, d = 8, p = 0.2, seed = 11): the first 2^22 rows bit for
+bit against the sequential oracle, and every row through the
+row-consistency verifier; the stats' node count.
 """
 import os
 
@@ -114,22 +114,23 @@ def test_full_degree_histogram(ctx, big_graph):
     assert int((h * x).sum()) == d * (d + 1) + 2 * (n - d - 1) * d
 
 
-def test_n2p30_sampled_rows_consistent(ctx, big_graph, oracle_mod):
+@pytest.mark.timeout(900)
+def test_n2p30_every_row_consistent(ctx, big_graph, oracle_mod):
     import torch
     n, d, p, seed = 1 << 30, 8, 0.2, 11
     T = oracle_mod.p_threshold(p)
     print(f"\n[scale] generating n={n} d={d} p={p} seed={seed}", flush=True)
     g = ctx.generate(n, d, p, seed, row_ptr=False)
+    assert ctx.stats()["nodes"] == n - d - 1
     host = g.targets.cpu().numpy()
     del g
     torch.cuda.empty_cache()
-    ranges = [(0, 1 << 22), (n - (1 << 16), 1 << 16)]
-    for i in range(64):
-        x = oracle_mod.philox((i, 0xBEEF, 0, 0), (seed, 0))
-        ranges.append((int(((x[0] << 32) | x[1]) % (n - (1 << 16))), 1 << 16))
-    for start, width in ranges:
-        bad = oracle_mod.verify_rows(n, d, T, seed, host, start, width, threads=THREADS)
-        assert bad is None, f"row {bad} inconsistent (range start {start})"
+    want, _, _ = oracle_mod.generate(1 << 22, d, T, seed)
+    assert np.array_equal(host[:(1 << 22) * d], want)
+    del want
+    print("[scale] verifying every row of n=2^30", flush=True)
+    bad = oracle_mod.verify_rows(n, d, T, seed, host, threads=THREADS)
+    assert bad is None, f"first inconsistent row {bad}"
 
 
 def test_d16_n2p24_every_row_consistent(ctx, oracle_mod):

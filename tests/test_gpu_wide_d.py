@@ -27,6 +27,36 @@ def test_wide_d_bit_exact(ctx, oracle_mod, n, d, p, seed):
     assert st["nodes"] == n - d - 1 and st["retries"] == want_retries
 
 
+@pytest.mark.parametrize("n,d,p,seed,tuning", [
+    # every row through the LDS-staged publication (no slot-by-slot early
+    # phase): 4-byte pieces for odd d, 16-byte pieces with 5..8 per lane for
+    # d = 20..32
+    (60_000, 3, 0.3, 31, dict(early_nodes=1)),
+    (60_000, 5, 0.1, 32, dict(early_nodes=1)),
+    (40_000, 17, 0.3, 33, dict(early_nodes=1)),
+    (40_000, 20, 0.2, 34, dict(early_nodes=1)),
+    (30_000, 31, 0.5, 35, dict(early_nodes=1)),
+    (30_000, 32, 0.1, 36, dict(early_nodes=1)),
+    # one workgroup: rows finish far apart, so the three LDS flush windows
+    # overflow and the rest are stored by their own lanes
+    (20_000, 32, 0.2, 37, dict(early_nodes=1, max_blocks=1, block_threads=64)),
+    (20_000, 7, 0.2, 38, dict(early_nodes=1, max_blocks=1, block_threads=64)),
+])
+def test_staged_publication_bit_exact(ctx, oracle_mod, n, d, p, seed, tuning):
+    T = oracle_mod.p_threshold(p)
+    want, _, want_retries = oracle_mod.generate(n, d, T, seed)
+    try:
+        ctx.set_tuning(watchdog_ms=20000, **tuning)
+        g = ctx.generate(n, d, p_thresh=T, seed=seed)
+        st = ctx.stats()
+    finally:
+        ctx.set_tuning(watchdog_ms=20000)
+    assert np.array_equal(g.targets.cpu().numpy(), want)
+    assert st["nodes"] == n - d - 1 and st["retries"] == want_retries
+    # the staged path published rows
+    assert st["staged_rows"] > 0
+
+
 def test_wide_d_barabasi_albert(ctx, oracle_mod):
     import cuppa
     n, d, seed = 40_000, 24, 7
