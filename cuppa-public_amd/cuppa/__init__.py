@@ -107,6 +107,7 @@ _SIGNATURES = [
     ("ppa_get_stats", ctypes.c_int, [_VP, ctypes.POINTER(Stats)]),
     ("ppa_generate", ctypes.c_int, [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP]),
     ("ppa_generate_async", ctypes.c_int, [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, _VP]),
+    ("ppa_generate_noalloc", ctypes.c_int, [ctypes.POINTER(GraphDesc), _VP, _VP, ctypes.POINTER(Stats), _VP]),
     ("ppa_ctx_sync", ctypes.c_int, [_VP, _VP]),
     ("ppa_generate_range", ctypes.c_int,
      [_VP, ctypes.POINTER(GraphDesc), _VP, _VP, ctypes.c_uint64, ctypes.c_uint64, _VP]),

@@ -168,6 +168,21 @@ int ppa_get_stats(ppa_ctx *ctx, ppa_stats *out);
  * returns the kernel's status. */
 int ppa_generate(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets,
                  int64_t *d_row_ptr, void *stream);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ppa_generate on the current device without a context,
+ * with the default launch rules. It allocates nothing: the launch's claim
+ * counter, status word and counters (128 bytes) live in a static device
+ * buffer of 256 bytes that the library holds on each GPU it runs on until the
+ * process ends. That record is all the generator needs whatever n and d are,
+ * so no valid graph exceeds it; descriptors are checked as ppa_generate
+ * checks them. Each call resets the record before its kernels run. Calls that
+ * overlap on one GPU share the record and race: issue them one at a time per
+ * GPU. Enqueues on `stream`, waits, and returns the kernel's status; stats
+ * (nullable) receives the counters (kernel_ms and total_ms are 0: no events
+ * are created). */
+int ppa_generate_noalloc(const ppa_graph_desc *desc, int32_t *d_targets, int64_t *d_row_ptr,
+                         ppa_stats *stats, void *stream);
 /* Same work without the final wait; the status is read by ppa_ctx_sync. Until
  * then the launch owns the context's scratch: another ppa_generate*,
  * ppa_batch_launch, ppa_generate_batch, ppa_autotune, ppa_degree_histogram,
