@@ -165,8 +165,9 @@ def gather_ceiling(torch, table_bytes: int = 32 << 30, reads: int = 1 << 28) -> 
         blocks, threads, iters = 2048, 256, 512
         sink = torch.empty(blocks * threads, dtype=torch.int32, device="cuda")
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-        for mode, name in enumerate(("probe_plain", "probe_nt", "probe_lds", "probe_lds_nt",
-                                     "probe_lds_nt_with_row_stores")):
+        for mode, name in enumerate(PROBE_MODES):
+            if name == "philox_draws":
+                continue
             best = float("inf")
             for _ in range(4):  # the first launch warms up
                 ms = ctypes.c_float()
@@ -193,6 +194,49 @@ def gather_ceiling(torch, table_bytes: int = 32 << 30, reads: int = 1 << 28) -> 
     torch.cuda.empty_cache()
     out["requests_per_s"] = max(v for k, v in out.items() if k not in ("table_bytes", "probe_lds_nt_with_row_stores"))
     return out
+
+
+# ppa_probe_gather's modes, in order (mode i = PROBE_MODES[i])
+PROBE_MODES = ("probe_plain", "probe_nt", "probe_lds", "probe_lds_nt", "probe_lds_nt_with_row_stores",
+               "probe_lds_sc0", "probe_lds_sc1", "probe_lds_sc0_sc1", "probe_lds_nt_sc0", "probe_lds_nt_sc1",
+               "probe_lds_nt_sc0_sc1", "philox_draws")
+
+
+def philox_roof(torch, blocks: int = 8192, threads: int = 256, iters: int = 4096) -> dict:
+    """The integer-VALU roof of the generator's draws: keyed Philox-4x32-10
+    attempts per second (coin and mulhi64 pair mapping included) with no
+    memory traffic, the whole GPU busy (probe mode 11, best of 3)."""
+    import ctypes
+    probe_path = os.path.join(ROOT, "cuppa-public_amd", "lib", "libppa_probe.so")
+    if not os.path.exists(probe_path):
+        return None
+    P = ctypes.CDLL(probe_path)
+    P.ppa_probe_gather.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                   ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.POINTER(ctypes.c_float)]
+    P.ppa_probe_gather.restype = ctypes.c_int
+    sink = torch.empty(blocks * threads, dtype=torch.int32, device="cuda")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    best = float("inf")
+    for _ in range(4):
+        ms = ctypes.c_float()
+        rc = P.ppa_probe_gather(ctypes.c_void_p(sink.data_ptr()), sink.numel(), blocks, threads, iters,
+                                PROBE_MODES.index("philox_draws"), ctypes.c_void_p(sink.data_ptr()), stream,
+                                ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError(f"ppa_probe_gather failed: {rc}")
+        best = min(best, ms.value)
+    del sink
+    return {"draws_per_s": blocks * threads * iters / (best * 1e-3), "lanes": blocks * threads,
+            "draws_per_lane": iters}
+
+
+def run_probe(args):
+    """Each probe kernel once over a 32 GiB table, for counter runs
+    (rocprofv3 --pmc): the gather forms and the Philox draws."""
+    import torch
+    out = {"gather": gather_ceiling(torch), "philox": philox_roof(torch)}
+    print(json.dumps(out))
 
 
 def read_requests(profile_dir: str, workload: str):
@@ -484,7 +528,7 @@ def main():
                          "and the reference's serial BA on a prefix")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB three times (counter calibration) and exit")
-    ap.add_argument("--mode", default="graph", choices=("graph", "batch", "serve", "latency"),
+    ap.add_argument("--mode", default="graph", choices=("graph", "batch", "serve", "latency", "probe"),
                     help="graph: the single-graph metric with the batch and serve objects (default); "
                          "batch / serve: one of those objects alone; latency: single-call wall times of "
                          "small and mid-size graphs")
@@ -535,6 +579,8 @@ def main():
         return run_serve(args)
     if args.mode == "latency":
         return run_latency(args)
+    if args.mode == "probe":
+        return run_probe(args)
 
     n, d, p, seed = WORKLOADS[args.workload]
     ctx = cuppa.Context(0)
@@ -619,6 +665,11 @@ def main():
         if "probe_lds_nt_with_row_stores" in ceil:
             # against reads that share the memory with row stores, as the generator's do
             request_roof["frac_of_read_write_mix"] = rate / ceil["probe_lds_nt_with_row_stores"]
+    roof = philox_roof(torch)
+    if roof:
+        # the generator's own draws per second against that roof
+        roof["generator_draws_per_s"] = last["draws"] / (k_ms * 1e-3)
+        roof["frac"] = roof["generator_draws_per_s"] / roof["draws_per_s"]
     cpu = cpu_baseline(n, d, p, args.cpu_prefix) if args.cpu_prefix else None
     if not args.no_batch:
         extra["batch"] = batch_object(args, ctx)
@@ -653,6 +704,8 @@ def main():
         # the kernel's 64-byte read requests (committed PMC profile) per second
         # against a plain random gather's on the same GPU
         "request_roofline": request_roof,
+        # the integer-VALU roof of the draws (Philox alone on the whole GPU)
+        "philox_roof": roof,
         "call_roofline_frac": bpe["call"] * edges / (c_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "call_ms": c_ms,
         "measured_copy_gbs": copy_gbs,
