@@ -78,3 +78,39 @@ def test_histogram_of_degrees(ctx):
             w[:k] = want[:k]
             assert np.array_equal(h.cpu().numpy(), w)
     assert L.ppa_histogram_of_degrees(ctx.handle, n, None, None, 0, None, None) == -1
+
+
+def test_scratch_budget_fallback_and_trim(ctx, oracle_mod):
+    """A context whose scratch budget is below the partition buffers (one
+    int32 copy of the slots at this n, two from n > 2^22 on) counts with one
+    atomic per slot instead, holds no work memory, and gets the same degrees;
+    its in-edge CSR is refused with PPA_ERR_BUDGET. A context without a cap
+    keeps the buffers until ppa_ctx_trim."""
+    import torch
+    import cuppa
+    n, d, p, seed = 300_000, 8, 0.3, 17
+    T = oracle_mod.p_threshold(p)
+    G, _, _ = oracle_mod.generate(n, d, T, seed)
+    deg_want, _, mx_want = oracle_mod.degrees(n, d, G)
+    hist_want = np.bincount(deg_want.astype(np.int64))
+    small = cuppa.Context(0, scratch_budget=4 * n * d)
+    try:
+        assert small.memory() == {"held": 0, "budget": 4 * n * d}
+        g = small.generate(n, d, p_thresh=T, seed=seed)
+        hist, mx, deg = small.degree_histogram(g, want_degree=True)
+        assert small.memory()["held"] == 0  # the atomic kernels ran: no partition buffers
+        assert np.array_equal(deg.cpu().numpy().astype(np.uint64), deg_want)
+        assert mx == mx_want
+        assert np.array_equal(hist.cpu().numpy(), hist_want)
+        with pytest.raises(cuppa.PPAError) as e:
+            small.in_csr(g)
+        assert e.value.code == -6  # PPA_ERR_BUDGET
+    finally:
+        small.close()
+    g = ctx.generate(n, d, p_thresh=T, seed=seed)
+    hist, mx, deg = ctx.degree_histogram(g, want_degree=True)
+    assert ctx.memory()["held"] >= 4 * (n - d - 1) * d
+    ctx.trim()
+    assert ctx.memory()["held"] == 0
+    hist2, mx2, deg2 = ctx.degree_histogram(g, want_degree=True)  # works again after the trim
+    assert torch.equal(deg, deg2) and torch.equal(hist, hist2) and mx == mx2 == mx_want
