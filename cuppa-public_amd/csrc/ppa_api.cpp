@@ -23,10 +23,37 @@ bool tuning_valid(const ppa_tuning &t)
     return true;
 }
 
+// d > kNarrowMaxD: one 64-thread workgroup per node in flight, as many as
+// the CUs hold (the hash set's LDS limits that at large d), but no more than
+// the graph has claims. A claim is wide_claim_nodes(d) consecutive nodes.
+static int resolve_wide_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *out)
+{
+    const ppa_tuning &tu = ctx->tuning;
+    int per_cu = 0;
+    PPA_HIP(generate_occupancy(d, kWideThreads, &per_cu));
+    if (per_cu < 1) per_cu = 1;
+    if (tu.blocks_per_cu && (int)tu.blocks_per_cu < per_cu) per_cu = (int)tu.blocks_per_cu;
+    const uint32_t per_claim = wide_claim_nodes(d);
+    const uint64_t claims = (work + per_claim - 1) / per_claim;
+    uint64_t g = std::min<uint64_t>((uint64_t)per_cu * (uint64_t)ctx->num_cus, claims);
+    if (tu.max_blocks && g > tu.max_blocks) g = tu.max_blocks;
+    if (g < 1) g = 1;
+    out->blocks = (uint32_t)g;
+    out->threads = kWideThreads;
+    out->chunk = per_claim;
+    out->early_nodes = 0;
+    const uint32_t wd = tu.watchdog_ms ? tu.watchdog_ms : kDefaultWatchdogMs;
+    out->watchdog_ticks = (uint64_t)wd * (uint64_t)ctx->wall_clock_khz;
+    out->sleep_cap = tu.sleep_cap ? tu.sleep_cap : (uint32_t)std::min<uint64_t>(32, std::max<uint64_t>(1, g / 256));
+    out->wide_mask = 0;
+    return PPA_OK;
+}
+
 int resolve_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *out, bool cap_single)
 {
     const ppa_tuning &tu = ctx->tuning;
     if (!tuning_valid(tu)) return PPA_ERR_INVALID;
+    if (d > kNarrowMaxD) return resolve_wide_shape(ctx, d, work, out);
     const uint32_t th = tu.block_threads ? tu.block_threads : kDefaultBlock;
     const uint32_t ch = tu.chunk_nodes ? tu.chunk_nodes : kDefaultChunk;
     int per_cu = 0;

@@ -5,12 +5,48 @@
 // (src/PA/SPA/SerialCopyModel3.cpp:11-107): one host thread walks the nodes in
 // order and fills each node's slots with redraw-until-unique. It consumes the
 // same keyed Philox draws as the GPU generator (ppa_model.hpp), so the two
-// produce identical graphs; the duplicate set is the row itself (d <= 16)
-// instead of the reference's BSTVertex (BSTVertex.h:189-235).
+// produce identical graphs. The duplicate set is the row itself for d <= 32
+// and an open-addressing hash set above, instead of the reference's
+// BSTVertex (BSTVertex.h:189-235).
+#include <algorithm>
 #include <cstring>
+#include <new>
+#include <vector>
 
 #include "../../include/ppa.h"
 #include "ppa_model.hpp"
+
+namespace {
+
+// A node's accepted targets for d > 32: linear probing over the smallest
+// power of two >= 2d words (at most half full), cleared per node.
+class RowSet {
+public:
+    explicit RowSet(uint32_t d)
+    {
+        while ((1u << bits_) < 2u * d) ++bits_;
+        w_.assign((size_t)1 << bits_, ppa::kEmpty);
+    }
+    void clear() { std::fill(w_.begin(), w_.end(), ppa::kEmpty); }
+    // false if v is already in the set
+    bool insert(int32_t v)
+    {
+        const uint32_t mask = (1u << bits_) - 1u;
+        for (uint32_t i = ((uint32_t)v * 0x9E3779B1u) >> (32u - bits_);; i = (i + 1u) & mask) {
+            if (w_[i] == v) return false;
+            if (w_[i] == ppa::kEmpty) {
+                w_[i] = v;
+                return true;
+            }
+        }
+    }
+
+private:
+    uint32_t bits_ = 1;
+    std::vector<int32_t> w_;
+};
+
+}  // namespace
 
 extern "C" {
 
@@ -62,17 +98,29 @@ int ppa_cpu_generate(const ppa_graph_desc *desc, int32_t *targets, int64_t *row_
     const uint32_t k0 = (uint32_t)desc->seed, k1 = (uint32_t)(desc->seed >> 32);
     for (uint64_t i = 0; i <= d; ++i)
         for (uint32_t j = 0; j < d; ++j) targets[i * d + j] = ppa::seed_target(desc->model, d, i, j);
+    RowSet *set = nullptr;
+    if (d > 32) {
+        set = new (std::nothrow) RowSet(d);
+        if (!set) return PPA_ERR_NOMEM;
+    }
     uint64_t retries = 0;
     for (uint64_t t = (uint64_t)d + 1; t < n; ++t) {
         int32_t *row = targets + t * d;
         const uint64_t td = t * d;
+        if (set) set->clear();
         for (uint32_t k = 0; k < d; ++k) {
             for (uint32_t r = 0;; ++r) {
-                if (r >= PPA_MAX_RETRY) return PPA_ERR_RETRY;
+                if (r >= PPA_MAX_RETRY) {
+                    delete set;
+                    return PPA_ERR_RETRY;
+                }
                 const ppa::Attempt a = ppa::attempt(k0, k1, T, t, k, r, td);
                 const int32_t cand = a.direct ? (int32_t)(a.pair / d) : targets[a.pair];
                 bool dup = false;
-                for (uint32_t i = 0; i < k && !dup; ++i) dup = row[i] == cand;
+                if (set)
+                    dup = !set->insert(cand);
+                else
+                    for (uint32_t i = 0; i < k && !dup; ++i) dup = row[i] == cand;
                 if (!dup) {
                     row[k] = cand;
                     break;
@@ -81,6 +129,7 @@ int ppa_cpu_generate(const ppa_graph_desc *desc, int32_t *targets, int64_t *row_
             }
         }
     }
+    delete set;
     if (row_ptr)
         for (uint64_t t = 0; t <= n; ++t) row_ptr[t] = (int64_t)(t * d);
     if (retries_out) *retries_out = retries;

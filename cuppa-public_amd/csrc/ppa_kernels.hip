@@ -769,6 +769,203 @@ void generate_kernel(GenArgs a)
     }
 }
 
+// ---- out-degrees above 32 -------------------------------------------------
+// A node of d > 32 slots is a whole wavefront's work: d registers per lane
+// would not fit, and a node's d copy reads alone keep a wavefront's 64 lanes
+// busy. The reference bounds d by its 48 KB shared-memory wait queue of 6,144
+// entries per block (cuPPA_RRP_1.cu:49-56) and its int16 slot index
+// (Common/DataTypes.h:22-27); here d goes up to PPA_MAX_D. A wavefront claims a few consecutive nodes at a time (about 256
+// slots' worth, in ascending order) and walks each node's slots in windows of
+// 64, one slot per lane. The node's accepted targets live in an LDS hash set
+// (linear probing, at most half full, cleared per node), so a duplicate check
+// costs a few LDS reads instead of a scan of up to d - 1 earlier slots. One
+// round of a window:
+//   1. lanes whose slot needs an attempt draw it: a direct target is known at
+//      once, a copy draw reads its source slot (a plain read first -- slots
+//      are write-once, so it returns -1 or the final value -- and agent-scope
+//      re-reads while the source is still empty);
+//   2. the first open lane (from `acc`, the window's accepted slots) whose
+//      candidate is missing, already in the set, or equal to an earlier open
+//      lane's candidate ends the accepted run: the lanes before it insert
+//      their candidates into the set and publish their slots at once;
+//   3. every open lane whose candidate is now in the set draws its next
+//      attempt: it is rejected whatever the slots between decide.
+// The draws and the order of acceptance are the sequential model's, so the
+// graph is the same. The smallest unfinished node only waits on finished,
+// published slots, so the grid always makes progress.
+
+__device__ __forceinline__ uint32_t set_slot(int32_t v, uint32_t hbits)
+{
+    return ((uint32_t)v * 0x9E3779B1u) >> (32u - hbits);  // Fibonacci hashing; hbits >= 7
+}
+
+__device__ __forceinline__ bool set_contains(const int32_t *set, uint32_t hbits, int32_t v)
+{
+    const uint32_t mask = (1u << hbits) - 1u;
+    for (uint32_t i = set_slot(v, hbits);; i = (i + 1u) & mask) {
+        const int32_t e = __hip_atomic_load(&set[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (e == v) return true;
+        if (e == kEmpty) return false;  // the set is never full: the probe ends
+    }
+}
+
+// v is not in the set, and the lanes inserting together hold distinct values.
+__device__ __forceinline__ void set_insert(int32_t *set, uint32_t hbits, int32_t v)
+{
+    const uint32_t mask = (1u << hbits) - 1u;
+    for (uint32_t i = set_slot(v, hbits);; i = (i + 1u) & mask) {
+        int32_t none = kEmpty;
+        if (__hip_atomic_compare_exchange_strong(&set[i], &none, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP))
+            return;
+    }
+}
+
+// One wavefront per workgroup; the hash set (1 << hbits int32 words, at least
+// 2d) is the dynamic LDS.
+__global__ __launch_bounds__(64) void wide_generate_kernel(GenArgs a, uint32_t d, uint32_t hbits, double inv_d)
+{
+    extern __shared__ int32_t hset[];
+    const uint32_t lane = lane_id();
+    const uint32_t hsize = 1u << hbits;
+    const uint64_t start_clock = wall_clock64();
+    const uint32_t per_claim = a.chunk;  // wide_claim_nodes(d), >= 1
+    uint64_t s_nodes = 0, s_retries = 0, s_claims = 0, s_sleeps = 0, s_rounds = 0;
+    uint64_t c_draws = 0;
+    uint32_t c_pmiss = 0, c_polls = 0, c_stalls = 0;
+    uint32_t backoff = 1;
+    int fail = 0;
+    bool stop = false;
+
+    while (!stop) {
+        uint64_t base = 0;
+        if (lane == 0) base = atomicAdd(&a.scratch->claim, (unsigned long long)per_claim);
+        base = uniform64(base);
+        ++s_claims;
+        if (base >= a.n) break;
+        const uint64_t end = base + per_claim < a.n ? base + per_claim : a.n;
+        for (uint64_t t = base; t < end && !stop; ++t) {
+            for (uint32_t i = lane; i < hsize; i += kWave) hset[i] = kEmpty;
+            __syncthreads();
+            const uint64_t td = t * (uint64_t)d;
+            for (uint32_t w0 = 0; w0 < d && !stop; w0 += kWave) {
+                const uint32_t width = d - w0 < (uint32_t)kWave ? d - w0 : (uint32_t)kWave;
+                const bool mine = lane < width;
+                const uint32_t k = w0 + lane;
+                uint32_t r = 0;
+                bool need_draw = mine, pending = false, first_read = false, have = false;
+                uint64_t src = 0;
+                int32_t cand = kEmpty;
+                uint32_t acc = 0;  // wavefront-uniform: slots w0 .. w0+acc-1 are accepted
+                while (acc < width) {
+                    bool progressed = false, retried = false;
+                    if (need_draw) {
+                        const Attempt at = attempt(a.key0, a.key1, a.T, t, k, r, td);
+                        ++c_draws;
+                        need_draw = false;
+                        if (at.direct) {
+                            cand = (int32_t)div_small(at.pair, d, inv_d);
+                            have = true;
+                        } else {
+                            src = at.pair;
+                            pending = true;
+                            first_read = true;
+                        }
+                    }
+                    if (pending) {
+                        int32_t v;
+                        if (first_read) {
+                            v = a.G[src];
+                            first_read = false;
+                            if (v <= kEmpty) ++c_pmiss;
+                        } else {
+                            v = load_coherent(a.G + src);
+                            ++c_polls;
+                            if (v <= kEmpty) ++c_stalls;
+                        }
+                        if (v > kEmpty) {
+                            cand = v;
+                            have = true;
+                            pending = false;
+                            progressed = true;
+                        }
+                    }
+                    const bool open = mine && lane >= acc;
+                    const bool in_set = open && have && set_contains(hset, hbits, cand);
+                    // the accepted run cannot pass the first open lane still without a candidate
+                    const uint64_t wait_mask = __ballot(open && !have);
+                    const uint32_t lim = wait_mask ? (uint32_t)__builtin_ctzll(wait_mask) : width;
+                    bool dup_lane = false;
+                    for (uint32_t i = acc; i + 1u < lim; ++i) {
+                        const int32_t ci = __builtin_amdgcn_readlane(cand, (int)i);
+                        dup_lane |= lane > i && cand == ci;
+                    }
+                    const uint64_t bad_mask = __ballot(open && lane < lim && (in_set || dup_lane));
+                    const uint32_t first = bad_mask ? (uint32_t)__builtin_ctzll(bad_mask) : lim;
+                    if (open && lane < first) {
+                        set_insert(hset, hbits, cand);
+                        store_coherent(a.G + td + k, cand);
+                    }
+                    if (first > acc) progressed = true;
+                    acc = first;
+                    __syncthreads();
+                    if (mine && lane >= acc && have && set_contains(hset, hbits, cand)) {
+                        have = false;
+                        need_draw = true;
+                        retried = true;
+                        progressed = true;
+                        if (++r >= kMaxRetry) fail = kErrRetry;
+                    }
+                    s_retries += (uint64_t)__popcll(__ballot(retried));
+                    ++s_rounds;
+                    if (__ballot(fail != 0) != 0) {
+                        if (fail != 0) atomicCAS(&a.scratch->status, 0, fail);
+                        stop = true;
+                        break;
+                    }
+                    if ((s_rounds & 63u) == 0) {
+                        int st = 0;
+                        if (lane == 0) {
+                            st = __hip_atomic_load(&a.scratch->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            if (st == 0 && wall_clock64() - start_clock > a.watchdog_ticks) {
+                                atomicCAS(&a.scratch->status, 0, kErrTimeout);
+                                st = kErrTimeout;
+                            }
+                        }
+                        if (__builtin_amdgcn_readfirstlane(st) != 0) {
+                            stop = true;
+                            break;
+                        }
+                    }
+                    if (__ballot(progressed) == 0) {
+                        // every open slot waits on another node's slot
+                        ++s_sleeps;
+                        for (uint32_t s = 0; s < backoff; ++s) __builtin_amdgcn_s_sleep(4);
+                        backoff = backoff * 2 < a.sleep_cap ? backoff * 2 : a.sleep_cap;
+                    } else {
+                        backoff = 1;
+                    }
+                }
+            }
+            if (!stop) ++s_nodes;
+            __syncthreads();  // the set's last lookups before the next node clears it
+        }
+    }
+
+    const uint64_t draws = wave_sum64(c_draws);
+    const uint64_t pmiss = wave_sum64(c_pmiss), polls = wave_sum64(c_polls), stalls = wave_sum64(c_stalls);
+    if (lane == 0) {
+        unsigned long long *st = a.scratch->stats;
+        const uint64_t v[10] = {s_nodes, s_nodes * d, draws, s_retries, pmiss, polls, stalls, s_claims, s_sleeps,
+                                s_rounds};
+        const int idx[10] = {kStatNodes,  kStatSlots,  kStatDraws,  kStatRetries, kStatPrefetchMisses,
+                             kStatPolls,  kStatStalls, kStatClaims, kStatSleeps,  kStatPasses};
+#pragma unroll
+        for (int i = 0; i < 10; ++i)
+            if (v[i]) atomicAdd(&st[idx[i]], (unsigned long long)v[i]);
+    }
+}
+
 // ---- LDS-resident prefix ----------------------------------------------------
 // A small graph's run time is its longest dependency chain times the cost of
 // one dependent step, and most of that chain lies among the first nodes,
@@ -1074,8 +1271,7 @@ __global__ void degree_count_kernel(const int32_t *G, uint64_t first_slot, uint6
 // Barabasi-Albert count (PAHelper.hpp:84-121) leaves out their self-loops.
 __global__ void seed_degree_kernel(uint32_t *deg, uint32_t d, uint32_t model)
 {
-    const uint32_t v = threadIdx.x;
-    if (v < d) deg[v] = deg[v] - d + base_degree(model, d, v);
+    for (uint32_t v = threadIdx.x; v < d; v += blockDim.x) deg[v] = deg[v] - d + base_degree(model, d, v);
 }
 
 constexpr int kHistLds = 4096;
@@ -1153,6 +1349,17 @@ hipError_t occupancy_d(uint32_t threads, int *out)
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, generate_kernel<D, BATCH>, (int)threads, 0);
 }
 
+hipError_t launch_generate_wide(uint32_t d, const GenArgs &a, uint32_t blocks, uint32_t threads,
+                                hipStream_t stream)
+{
+    // a.chunk nodes per claim: zero would never advance the claims
+    if (d <= kNarrowMaxD || d > PPA_MAX_D || threads != kWideThreads || a.chunk == 0) return hipErrorInvalidValue;
+    const uint32_t hbits = wide_hash_bits(d);
+    hipLaunchKernelGGL(wide_generate_kernel, dim3(blocks), dim3(kWideThreads), (size_t)4u << hbits, stream, a, d,
+                       hbits, 1.0 / d);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_reset_scratch(DeviceScratch *s, uint64_t first, hipStream_t stream)
@@ -1193,7 +1400,7 @@ hipError_t launch_generate(uint32_t d, const GenArgs &a, uint32_t blocks, uint32
     case DD: return launch_generate_d<DD, false>(a, blocks, threads, stream);
         PPA_FOR_EACH_D(PPA_CASE)
 #undef PPA_CASE
-    default: return hipErrorInvalidValue;
+    default: return launch_generate_wide(d, a, blocks, threads, stream);
     }
 }
 
@@ -1229,7 +1436,10 @@ hipError_t generate_occupancy(uint32_t d, uint32_t threads, int *out)
     case DD: return occupancy_d<DD, false>(threads, out);
         PPA_FOR_EACH_D(PPA_CASE)
 #undef PPA_CASE
-    default: return hipErrorInvalidValue;
+    default:
+        if (d > PPA_MAX_D || threads != kWideThreads) return hipErrorInvalidValue;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, wide_generate_kernel, (int)kWideThreads,
+                                                            (size_t)4u << wide_hash_bits(d));
     }
 }
 
@@ -1246,7 +1456,7 @@ hipError_t launch_degree_atomic(const int32_t *G, uint64_t n, uint32_t d, uint32
 hipError_t launch_seed_degree(uint32_t *deg, uint32_t d, uint32_t model, hipStream_t stream)
 {
     if (model != PPA_MODEL_COPY)
-        hipLaunchKernelGGL(seed_degree_kernel, dim3(1), dim3(64), 0, stream, deg, d, model);
+        hipLaunchKernelGGL(seed_degree_kernel, dim3(1), dim3(256), 0, stream, deg, d, model);
     return hipGetLastError();
 }
 

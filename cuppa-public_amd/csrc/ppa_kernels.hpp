@@ -66,7 +66,8 @@ struct GenArgs {
     const uint64_t *seg_active;
     uint64_t nseg;
     // common
-    uint32_t chunk;           // work claimed per wavefront claim (multiple of 64)
+    uint32_t chunk;           // work claimed per wavefront claim (multiple of 64; d > kNarrowMaxD:
+                              // nodes, wide_claim_nodes(d))
     uint32_t early_chunk;     // claim size below early_nodes (single graph)
     uint64_t early_nodes;     // below this node index every slot is published as accepted
     uint64_t watchdog_ticks;  // wall-clock ticks a wavefront may run
@@ -174,11 +175,27 @@ hipError_t launch_prefix_batch(const PrefixArgs &a, uint32_t count, hipStream_t 
 hipError_t launch_generate_batch(const GenArgs &a, uint32_t blocks, uint32_t block_threads, hipStream_t stream);
 hipError_t generate_occupancy(uint32_t d, uint32_t block_threads, int *blocks_per_cu);
 hipError_t launch_degree_atomic(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, hipStream_t stream);
-// Adjusts the seed nodes' degrees of a non-copy model (d <= 64 threads).
+// Adjusts the seed nodes' degrees of a non-copy model (one workgroup).
 hipError_t launch_seed_degree(uint32_t *deg, uint32_t d, uint32_t model, hipStream_t stream);
 hipError_t launch_degree_histogram(const uint32_t *deg, uint64_t n, uint64_t *hist, uint64_t hist_len,
                                    DeviceScratch *s, hipStream_t stream);
 
 constexpr uint32_t kBatchDMax = 16;  // d of 0 selects the batch kernel in generate_occupancy
+
+// Out-degrees above kNarrowMaxD run the wide generator: one wavefront (a
+// 64-thread workgroup) per node, whose accepted targets sit in an LDS hash set
+// of 1 << wide_hash_bits(d) int32 words, the smallest power of two >= 2d (at
+// most half full; 64 KB at d = 8192).
+constexpr uint32_t kNarrowMaxD = 32;
+constexpr uint32_t kWideThreads = 64;
+inline uint32_t wide_hash_bits(uint32_t d)
+{
+    uint32_t h = 1;
+    while ((1u << h) < 2u * d) ++h;
+    return h;
+}
+// Nodes a wavefront of the wide generator claims at once (GenArgs::chunk):
+// about 256 slots' worth, so that claims cost one atomic per 4+ rounds.
+inline uint32_t wide_claim_nodes(uint32_t d) { return d >= 256 ? 1u : (256u + d - 1u) / d; }
 
 }  // namespace ppa

@@ -20,7 +20,8 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libppa.so")
 
 PPA_OK = 0
-PPA_MAX_D = 32
+PPA_MAX_D = 8192
+PPA_NARROW_MAX_D = 32  # d above this runs the wavefront-per-node generator
 PPA_MAX_N = 0x7FFFFFFF
 PPA_MAX_RETRY = 1 << 24
 

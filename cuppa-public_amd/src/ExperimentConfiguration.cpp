@@ -103,7 +103,7 @@ void printUsage(const char *prog)
         "  -A,--algorithm NAME       cuppa | cuppa-adaptive | scm | sba\n"
         "  -l,--loadbalancing NAME   accepted (rrp is the only scheme of the reference)\n"
         "  -n,--n1 N                 number of vertices\n"
-        "  -x,--x1 M                 out-edges per new vertex (1..32)\n"
+        "  -x,--x1 M                 out-edges per new vertex (1..8192)\n"
         "  --weight1 W, -N,--n2 N, -X,--x2 M, --weight2 W   accepted, unused\n"
         "  -a,--alpha P              probability of creating a direct edge\n"
         "  -s,--stages S             accepted (one persistent launch covers all stages)\n"

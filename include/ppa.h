@@ -53,7 +53,10 @@ enum {
 };
 
 #define PPA_MAX_RETRY (1u << 24)
-#define PPA_MAX_D 32u            /* largest out-degree the generator supports */
+#define PPA_MAX_D 8192u          /* largest out-degree the generator supports; d <= 32 runs
+                                    the lane-per-node generator, larger d one wavefront per
+                                    node (the reference's 48 KB wait queue caps d at 6144,
+                                    cuPPA_RRP_1.cu:49-56) */
 #define PPA_MAX_BATCH_D 16u      /* largest out-degree of a batch's graphs and of requests the
                                     server may batch (larger ones always launch alone) */
 #define PPA_MAX_N 0x7fffffffull  /* targets are int32 (DataTypes.h:11-18) */
@@ -89,7 +92,10 @@ typedef struct ppa_graph_desc {
     uint64_t row_ptr_offset;
 } ppa_graph_desc;
 
-/* Launch shape of the generator. Any shape gives the same output. */
+/* Launch shape of the generator. Any shape gives the same output. For d > 32
+ * the generator runs one wavefront per node in 64-thread workgroups: there
+ * block_threads, chunk_nodes, early_nodes and prefix_nodes are ignored, and
+ * blocks_per_cu, max_blocks, watchdog_ms and sleep_cap apply as below. */
 typedef struct ppa_tuning {
     uint32_t block_threads;  /* workgroup size, multiple of 64, <= 256 (0 = default 256) */
     uint32_t blocks_per_cu;  /* resident workgroups per CU, at most the occupancy maximum (0 = that

@@ -47,6 +47,9 @@
 /* Retry cap per slot; the product applies the same cap. */
 #define ORACLE_MAX_RETRY (1u << 24)
 
+/* Largest out-degree the row verifier takes (the product's limit too). */
+#define ORACLE_MAX_D 8192u
+
 /* ---- Philox-4x32-10 (Salmon et al., SC'11) ------------------------------ */
 
 static const uint32_t PHILOX_M0 = 0xD2511F53u;
@@ -486,9 +489,9 @@ int oracle_verify_rows_model(uint32_t model, uint64_t n, uint32_t d, uint64_t T,
                              const int32_t *G, uint64_t first, uint64_t count, uint64_t *bad)
 {
     *bad = UINT64_MAX;
-    if (d == 0 || d > 4096 || n < (uint64_t)d + 1 || first + count > n || model > 1) return ORACLE_ERR_ARG;
+    if (d == 0 || d > ORACLE_MAX_D || n < (uint64_t)d + 1 || first + count > n || model > 1) return ORACLE_ERR_ARG;
     if (model == 1) T = 1ull << 31;
-    int32_t row[4096];
+    int32_t row[ORACLE_MAX_D];
     for (uint64_t t = first; t < first + count; ++t) {
         const int32_t *g = G + t * d;
         if (t <= d) {

@@ -144,6 +144,7 @@ int main()
         {2, 1, 0.5, 1},       {3, 2, 0.5, 2},          {17, 16, 0.3, 3},       {33, 32, 0.1, 4},
         {2000, 1, 0.0, 5},    {2000, 1, 1.0, 6},       {5000, 3, 0.0, 7},      {20000, 8, 0.2, 8},
         {6000, 16, 0.1, 9},   {3000, 32, 0.5, 10},     {10000, 5, 0.9, 1ull << 40},
+        {34, 33, 0.5, 11},    {2000, 33, 0.2, 12},     {1500, 100, 0.0, 13},   {1100, 1000, 0.7, 14},
     };
     for (const auto &c : cases) {
         one_graph(c.n, c.d, c.p, c.seed, PPA_MODEL_COPY);

@@ -66,7 +66,8 @@ def test_strerror_and_threshold(L, oracle_mod):
 
 @pytest.mark.parametrize("n,d,T,ok", [
     (10, 4, 0, True), (5, 4, 0, True), (4, 4, 0, False), (10, 0, 0, False), (100, 16, 0, True),
-    (100, 17, 0, True), (100, 32, 0, True), (100, 33, 0, False), (33, 32, 0, True), (32, 32, 0, False),
+    (100, 17, 0, True), (100, 32, 0, True), (100, 33, 0, True), (33, 32, 0, True), (32, 32, 0, False),
+    (8193, 8192, 0, True), (8192, 8192, 0, False), (9000, 8193, 0, False),
     (2**31 - 1, 8, 0, True), (2**31, 8, 0, False), (10, 2, 2**32, True),
     (10, 2, 2**32 + 1, False)])
 def test_check_desc(L, n, d, T, ok):

@@ -1,0 +1,166 @@
+"""Out-degrees 33..8192: the wavefront-per-node generator (one 64-lane
+wavefront per node, slots in windows of 64, duplicate checks in an LDS hash
+set), bit for bit against the oracle, its statistics against the oracle's
+retry count, every ABI path that reaches it (plain calls, ranges, one
+workgroup, the server's lone launches), the degree pass and both CSR forms at
+these degrees, and row-consistency checks of larger graphs."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,d,p,seed", [
+    (34, 33, 0.5, 1),            # the seed clique and one generated node
+    (2_000, 33, 0.0, 2),         # copies only
+    (20_000, 48, 0.3, 3),
+    (20_000, 64, 0.2, 4),        # one full window per node
+    (10_000, 65, 0.5, 5),        # a full window and a one-slot window
+    (8_000, 100, 0.1, 6),
+    (3_000, 257, 0.3, 7),        # one node per claim
+    (2_500, 1_000, 0.5, 8),
+    (2_100, 2_048, 0.0, 9),
+    (8_225, 8_192, 0.5, 10),     # PPA_MAX_D: a 64 KB hash set, coupon-collector redraws
+])
+def test_wavefront_d_bit_exact(ctx, oracle_mod, n, d, p, seed):
+    T = oracle_mod.p_threshold(p)
+    want, _, want_retries = oracle_mod.generate(n, d, T, seed)
+    print(f"wavefront generator n={n} d={d} p={p} seed={seed}", flush=True)
+    g = ctx.generate(n, d, p_thresh=T, seed=seed)
+    assert np.array_equal(g.targets.cpu().numpy(), want)
+    assert np.array_equal(g.row_ptr.cpu().numpy(), np.arange(n + 1, dtype=np.int64) * d)
+    st = ctx.stats()
+    assert st["nodes"] == n - d - 1 and st["slots"] == (n - d - 1) * d
+    assert st["retries"] == want_retries
+    # every attempt is one Philox draw: the accepted ones and the rejected ones
+    assert st["draws"] == st["slots"] + st["retries"]
+
+
+@pytest.mark.parametrize("n,d,p,seed,tuning", [
+    (3_000, 40, 0.3, 21, dict(max_blocks=1)),   # one wavefront walks every node
+    (6_000, 200, 0.2, 22, dict(max_blocks=3)),
+    (6_000, 90, 0.6, 23, dict(blocks_per_cu=1, sleep_cap=1)),
+    # block_threads, chunk_nodes, early_nodes and prefix_nodes do not apply
+    (6_000, 70, 0.4, 24, dict(block_threads=128, chunk_nodes=64, early_nodes=1, prefix_nodes=1)),
+])
+def test_wavefront_d_launch_shapes(ctx, oracle_mod, n, d, p, seed, tuning):
+    T = oracle_mod.p_threshold(p)
+    want, _, want_retries = oracle_mod.generate(n, d, T, seed)
+    print(f"wavefront generator n={n} d={d} tuning={tuning}", flush=True)
+    try:
+        ctx.set_tuning(watchdog_ms=20000, **tuning)
+        g = ctx.generate(n, d, p_thresh=T, seed=seed)
+        st = ctx.stats()
+    finally:
+        ctx.set_tuning(watchdog_ms=20000)
+    assert np.array_equal(g.targets.cpu().numpy(), want)
+    assert st["nodes"] == n - d - 1 and st["retries"] == want_retries
+
+
+@pytest.mark.parametrize("n,d,seed", [(3_000, 100, 41), (2_200, 2_000, 42)])
+def test_wavefront_d_barabasi_albert(ctx, oracle_mod, n, d, seed):
+    import cuppa
+    print(f"wavefront generator BA n={n} d={d}", flush=True)
+    g = ctx.generate(n, d, seed=seed, model=cuppa.MODEL_BA)
+    want, _ = oracle_mod.generate_ba(n, d, seed)
+    assert np.array_equal(g.targets.cpu().numpy(), want)
+    hist, mx, deg = ctx.degree_histogram(g, want_degree=True)
+    wdeg, _, wmx = oracle_mod.degrees_ba(n, d, want)
+    assert np.array_equal(deg.cpu().numpy().astype(np.uint64), wdeg)
+    assert mx == wmx
+
+
+def test_wavefront_d_degrees_and_csr_forms(ctx, oracle_mod):
+    n, d, p, seed = 20_000, 100, 0.4, 51
+    T = oracle_mod.p_threshold(p)
+    print(f"wavefront generator degrees/CSR n={n} d={d}", flush=True)
+    g = ctx.generate(n, d, p_thresh=T, seed=seed)
+    G = g.targets.cpu().numpy()
+    assert oracle_mod.verify_rows(n, d, T, seed, G, threads=8) is None
+    hist, mx, deg = ctx.degree_histogram(g, want_degree=True)
+    wdeg, _, wmx = oracle_mod.degrees(n, d, G)
+    assert np.array_equal(deg.cpu().numpy().astype(np.uint64), wdeg)
+    assert mx == wmx
+    assert np.array_equal(hist.cpu().numpy(), np.bincount(wdeg.astype(np.int64), minlength=len(hist)))
+    in_ptr, in_src = ctx.in_csr(g)
+    want_ptr, want_src = oracle_mod.in_csr(n, d, G)
+    assert np.array_equal(in_ptr.cpu().numpy(), want_ptr)
+    assert np.array_equal(in_src.cpu().numpy(), want_src)
+    ptr, adj = ctx.undirected_csr(g)
+    want_ptr, want_adj = oracle_mod.undirected_csr(n, d, G)
+    assert np.array_equal(ptr.cpu().numpy(), want_ptr)
+    assert np.array_equal(adj.cpu().numpy(), want_adj)
+
+
+def test_wavefront_d_in_ranges(ctx, oracle_mod):
+    import torch
+    import cuppa
+    n, d, p, seed = 12_000, 64, 0.2, 61
+    T = oracle_mod.p_threshold(p)
+    want, _, _ = oracle_mod.generate(n, d, T, seed)
+    desc = cuppa.make_desc(n, d, None, seed, T)
+    targets = torch.full((n * d,), -7, dtype=torch.int32, device="cuda")
+    for a, b in ((0, 100), (100, 4_000), (4_000, n)):
+        print(f"wavefront generator range [{a}, {b}) d={d}", flush=True)
+        ctx.generate_range(desc, targets, None, a, b)
+        ctx.sync()
+        assert np.array_equal(targets[:b * d].cpu().numpy(), want[:b * d]), (a, b)
+
+
+@pytest.mark.parametrize("n,d,p,seed", [
+    (1 << 20, 64, 0.2, 71),      # 6.7e7 slots: every row checked
+    (1 << 17, 1_024, 0.5, 72),   # 1.3e8 slots: sampled rows
+])
+def test_wavefront_d_rows_consistent_at_scale(ctx, oracle_mod, n, d, p, seed):
+    """Each checked row equals what the model produces from the graph's own
+    earlier rows; with every row checked that is the oracle's graph by
+    induction over the nodes."""
+    T = oracle_mod.p_threshold(p)
+    print(f"wavefront generator at scale n={n} d={d}", flush=True)
+    g = ctx.generate(n, d, p_thresh=T, seed=seed)
+    st = ctx.stats()
+    assert st["nodes"] == n - d - 1
+    G = g.targets.cpu().numpy()
+    del g
+    assert G.min() >= 0 and G.max() < n
+    if n * d * d <= (1 << 34):
+        assert oracle_mod.verify_rows(n, d, T, seed, G, threads=16) is None
+    else:
+        rng = np.random.default_rng(seed)
+        starts = [d + 1, n - 64] + [int(x) for x in rng.integers(d + 1, n - 64, 14)]
+        for s in starts:
+            assert oracle_mod.verify_rows(n, d, T, seed, G, first=s, count=64, threads=16) is None, s
+    # rows hold distinct targets below their own node
+    rows = G.reshape(n, d)[d + 1:]
+    sub = rows[:: max(1, len(rows) // 2000)]
+    assert all(len(np.unique(r)) == d for r in sub)
+    assert (sub < np.arange(d + 1, n, max(1, len(rows) // 2000))[: len(sub), None]).all()
+
+
+def test_server_runs_wavefront_d_alone(ctx, oracle_mod):
+    import cuppa
+    rng = np.random.default_rng(81)
+    specs = []
+    for i in range(12):
+        d = int(rng.choice([8, 40, 100, 300]))
+        n = int(rng.integers(d + 2, 6_000))
+        specs.append((i, n, d, oracle_mod.p_threshold(0.3), 900 + i))
+    srv = cuppa.Server(ctx, budget_bytes=1 << 30, streams=4, batch_nodes=1 << 16, solo_nodes=1 << 20)
+    for s in specs:
+        assert srv.submit(*s) == 0
+    pending = {s[0]: s for s in specs}
+    for _ in range(2_000_000):
+        srv.step()
+        for rid in list(pending):
+            r = srv.poll(rid)
+            if r.state in (cuppa.REQ_DONE, cuppa.REQ_FAILED):
+                assert r.state == cuppa.REQ_DONE, (rid, r.status)
+                _, n, d, T, seed = pending.pop(rid)
+                got = cuppa.read_device(r.targets, n * d, np.int32)
+                want, _, _ = oracle_mod.generate(n, d, T, seed)
+                assert np.array_equal(got, want), (rid, n, d)
+                assert srv.release(rid) == 0
+        if not pending:
+            break
+    srv.close()
+    assert not pending

@@ -1,4 +1,4 @@
-"""Out-degrees 17..32 (up to PPA_MAX_D): single graphs through the LDS prefix
+"""Out-degrees 17..32 (the lane-per-node generator's largest): single graphs through the LDS prefix
 (32-lane groups, duplicate checks through ds_bpermute) and the persistent
 generator, bit for bit against the oracle; their degree pass, in-edge and
 undirected CSR and ranged generation. Batches refuse them, and the server

@@ -41,12 +41,31 @@ def test_noalloc_matches_generate_twice(ctx, oracle_mod):
     assert np.array_equal(got.cpu().numpy(), ref)
 
 
+def test_noalloc_wavefront_d_twice(ctx, oracle_mod):
+    """Out-degrees above 32 (one wavefront per node) through the same record."""
+    import torch
+    import cuppa
+    for n, d, p, seed in ((20_000, 100, 0.3, 73), (3_000, 40, 0.5, 74)):
+        desc = cuppa.make_desc(n, d, p, seed)
+        want = torch.full((n * d,), -7, dtype=torch.int32, device="cuda")
+        ctx.generate_into(desc, want, None)
+        got = torch.full((n * d,), -9, dtype=torch.int32, device="cuda")
+        got_rp = torch.full((n + 1,), -9, dtype=torch.int64, device="cuda")
+        rc, st = _noalloc(desc, got, got_rp)
+        assert rc == 0, cuppa.strerror(rc)
+        assert torch.equal(got, want), (n, d)
+        assert st.nodes == n - d - 1 and st.slots == (n - d - 1) * d
+        assert st.retries == ctx.stats()["retries"]
+    ref, _, _ = oracle_mod.generate(3_000, 40, oracle_mod.p_threshold(0.5), 74)
+    assert np.array_equal(got.cpu().numpy(), ref)
+
+
 def test_noalloc_rejects_bad_descriptors(ctx):
     import torch
     import cuppa
     t = torch.empty(1024, dtype=torch.int32, device="cuda")
     rp = torch.empty(65, dtype=torch.int64, device="cuda")
-    for desc in (cuppa.make_desc(64, 33, 0.5, 1), cuppa.make_desc(8, 8, 0.5, 1)):
+    for desc in (cuppa.make_desc(9000, 8193, 0.5, 1), cuppa.make_desc(8, 8, 0.5, 1)):
         rc, _ = _noalloc(desc, t, rp)
         assert rc == -1  # PPA_ERR_INVALID
     rc = cuppa.lib().ppa_generate_noalloc(ctypes.byref(cuppa.make_desc(64, 8, 0.5, 1)), None, None, None,
