@@ -774,16 +774,21 @@ void generate_kernel(GenArgs a)
 // would not fit, and a node's d copy reads alone keep a wavefront's 64 lanes
 // busy. The reference bounds d by its 48 KB shared-memory wait queue of 6,144
 // entries per block (cuPPA_RRP_1.cu:49-56) and its int16 slot index
-// (Common/DataTypes.h:22-27); here d goes up to PPA_MAX_D. A wavefront claims a few consecutive nodes at a time (about 256
-// slots' worth, in ascending order) and walks each node's slots in windows of
-// 64, one slot per lane. The node's accepted targets live in an LDS hash set
-// (linear probing, at most half full, cleared per node), so a duplicate check
-// costs a few LDS reads instead of a scan of up to d - 1 earlier slots. One
-// round of a window:
+// (Common/DataTypes.h:22-27); here d goes up to PPA_MAX_D. A wavefront
+// claims a few consecutive nodes at a time (about 256 slots' worth, in
+// ascending order) and walks each node's slots in windows of 64, one slot per
+// lane. The node's accepted targets live in an LDS hash set (linear probing,
+// at most half full, cleared per node), so a duplicate check costs a few LDS
+// reads instead of a scan of up to d - 1 earlier slots. One round of a window:
 //   1. lanes whose slot needs an attempt draw it: a direct target is known at
-//      once, a copy draw reads its source slot (a plain read first -- slots
-//      are write-once, so it returns -1 or the final value -- and agent-scope
-//      re-reads while the source is still empty);
+//      once, a copy draw reads its source slot straight into LDS (a plain read
+//      first -- slots are write-once, so it returns -1 or the final value --
+//      and agent-scope re-reads while the source is still empty). A slot
+//      rejected before draws its next kb attempts at once (kb doubling up to
+//      16 or 64 rows) and takes the first not already in the set: a node whose t
+//      candidates barely exceed its d slots (t close to d) has a coupon
+//      collector's tail of thousands of attempts in its last slots, and each
+//      of them would otherwise cost a memory round trip of its own;
 //   2. the first open lane (from `acc`, the window's accepted slots) whose
 //      candidate is missing, already in the set, or equal to an earlier open
 //      lane's candidate ends the accepted run: the lanes before it insert
@@ -821,17 +826,54 @@ __device__ __forceinline__ void set_insert(int32_t *set, uint32_t hbits, int32_t
     }
 }
 
-// One wavefront per workgroup; the hash set (1 << hbits int32 words, at least
-// 2d) is the dynamic LDS.
-__global__ __launch_bounds__(64) void wide_generate_kernel(GenArgs a, uint32_t d, uint32_t hbits, double inv_d)
+// Rows of attempts drawn at once (kb's cap): 64 for d >= 512, whose first
+// nodes have the longest retry tails (the last slot of node d + 1 takes about
+// d / 2 attempts), 16 below, where the LDS goes to occupancy instead.
+uint32_t wide_spec_rows(uint32_t d) { return d >= 512 ? 64u : 16u; }
+
+// The window table: for each candidate value of a round's open lanes, the
+// lowest lane holding it (128 keys, at most 64 in use, cleared per round).
+constexpr uint32_t kWinBits = 7;
+
+size_t wide_lds_bytes(uint32_t d)
 {
-    extern __shared__ int32_t hset[];
+    return (((size_t)1 << wide_hash_bits(d)) + 2u * (1u << kWinBits) + (size_t)wide_spec_rows(d) * kWave) * 4u;
+}
+
+// The slot of v in the window table, inserting it if absent.
+__device__ __forceinline__ uint32_t win_insert(int32_t *keys, int32_t v)
+{
+    const uint32_t mask = (1u << kWinBits) - 1u;
+    for (uint32_t i = set_slot(v, kWinBits);; i = (i + 1u) & mask) {
+        int32_t seen = kEmpty;
+        if (__hip_atomic_compare_exchange_strong(&keys[i], &seen, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP) ||
+            seen == v)
+            return i;
+    }
+}
+
+// One wavefront per workgroup. Dynamic LDS: the hash set (1 << hbits int32
+// words, at least 2d), the window table's keys and lowest lanes, then srows
+// rows of 64 words where attempt r + j of lane l lands in word l of row j.
+__global__ __launch_bounds__(64) void wide_generate_kernel(GenArgs a, uint32_t d, uint32_t hbits, uint32_t srows,
+                                                           double inv_d)
+{
+    extern __shared__ int32_t wide_lds[];
+    int32_t *const hset = wide_lds;
+    int32_t *const wkey = hset + (1u << hbits);
+    uint32_t *const wlane = reinterpret_cast<uint32_t *>(wkey + (1u << kWinBits));
+    int32_t *const spec = wkey + 2u * (1u << kWinBits);
     const uint32_t lane = lane_id();
     const uint32_t hsize = 1u << hbits;
     const uint64_t start_clock = wall_clock64();
+    for (uint32_t i = lane; i < (1u << kWinBits); i += kWave) {
+        wkey[i] = kEmpty;
+        wlane[i] = kWave;
+    }
     const uint32_t per_claim = a.chunk;  // wide_claim_nodes(d), >= 1
-    uint64_t s_nodes = 0, s_retries = 0, s_claims = 0, s_sleeps = 0, s_rounds = 0;
-    uint64_t c_draws = 0;
+    uint64_t s_nodes = 0, s_claims = 0, s_sleeps = 0, s_rounds = 0;
+    uint64_t c_draws = 0, c_retries = 0;
     uint32_t c_pmiss = 0, c_polls = 0, c_stalls = 0;
     uint32_t backoff = 1;
     int fail = 0;
@@ -852,71 +894,128 @@ __global__ __launch_bounds__(64) void wide_generate_kernel(GenArgs a, uint32_t d
                 const uint32_t width = d - w0 < (uint32_t)kWave ? d - w0 : (uint32_t)kWave;
                 const bool mine = lane < width;
                 const uint32_t k = w0 + lane;
-                uint32_t r = 0;
-                bool need_draw = mine, pending = false, first_read = false, have = false;
+                // r: the slot's current attempt (its candidate or pending
+                // source), or the next one to draw; kb: attempts the next draw takes
+                uint32_t r = 0, kb = 1;
+                // checked: cand is known not to be in the set as it stands
+                bool need_draw = mine, pending = false, have = false, checked = false;
                 uint64_t src = 0;
                 int32_t cand = kEmpty;
                 uint32_t acc = 0;  // wavefront-uniform: slots w0 .. w0+acc-1 are accepted
                 while (acc < width) {
-                    bool progressed = false, retried = false;
-                    if (need_draw) {
-                        const Attempt at = attempt(a.key0, a.key1, a.T, t, k, r, td);
-                        ++c_draws;
-                        need_draw = false;
-                        if (at.direct) {
-                            cand = (int32_t)div_small(at.pair, d, inv_d);
-                            have = true;
-                        } else {
-                            src = at.pair;
-                            pending = true;
-                            first_read = true;
+                    bool progressed = false;
+                    // ---- draws (attempt r + j into row j) and re-reads (row 0), all in flight at once ----
+                    for (uint32_t j = 0; j < srows; ++j) {
+                        const bool dj = need_draw && j < kb;
+                        if (__ballot(dj) == 0) break;
+                        if (dj) {
+                            const Attempt at = attempt(a.key0, a.key1, a.T, t, k, r + j, td);
+                            ++c_draws;
+                            if (at.direct)
+                                spec[j * kWave + lane] = (int32_t)div_small(at.pair, d, inv_d);
+                            else
+                                __builtin_amdgcn_global_load_lds(
+                                    (const __attribute__((address_space(1))) void *)(a.G + at.pair),
+                                    (__attribute__((address_space(3))) void *)(spec + j * kWave), 4, 0, 0);
                         }
                     }
                     if (pending) {
-                        int32_t v;
-                        if (first_read) {
-                            v = a.G[src];
-                            first_read = false;
-                            if (v <= kEmpty) ++c_pmiss;
-                        } else {
-                            v = load_coherent(a.G + src);
-                            ++c_polls;
-                            if (v <= kEmpty) ++c_stalls;
+                        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(a.G + src),
+                                                         (__attribute__((address_space(3))) void *)spec, 4, 0, kSc1);
+                        ++c_polls;
+                    }
+                    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+                    __syncthreads();
+                    if (need_draw) {
+                        // the first drawn attempt not already rejected by the set
+                        need_draw = false;
+                        progressed = true;
+                        uint32_t j = 0;
+                        bool done = false;
+                        while (!done && j < kb) {
+                            if (r + j >= kMaxRetry) {
+                                fail = kErrRetry;
+                                done = true;
+                                break;
+                            }
+                            const int32_t v = spec[j * kWave + lane];
+                            if (v <= kEmpty) {
+                                // its copy source is still empty: re-derive the source and wait on it
+                                ++c_pmiss;
+                                src = attempt(a.key0, a.key1, a.T, t, k, r + j, td).pair;
+                                pending = true;
+                                done = true;
+                            } else if (set_contains(hset, hbits, v)) {
+                                ++c_retries;  // it repeats an accepted slot: rejected
+                                ++j;
+                            } else {
+                                cand = v;
+                                have = true;
+                                checked = true;
+                                done = true;
+                            }
                         }
+                        r += j;
+                        if (!done) {
+                            need_draw = true;
+                            kb = kb * 2 < srows ? kb * 2 : srows;
+                        }
+                    } else if (pending) {
+                        const int32_t v = spec[lane];
                         if (v > kEmpty) {
                             cand = v;
                             have = true;
+                            checked = false;
                             pending = false;
                             progressed = true;
+                        } else {
+                            ++c_stalls;
                         }
                     }
+                    // ---- accept the run from acc up to the first slot that must wait or is rejected ----
                     const bool open = mine && lane >= acc;
-                    const bool in_set = open && have && set_contains(hset, hbits, cand);
+                    const bool in_set = open && have && !checked && set_contains(hset, hbits, cand);
+                    // the lowest open lane holding each candidate value
+                    const bool in_win = open && have && !in_set;
+                    uint32_t wslot = 0;
+                    if (in_win) {
+                        wslot = win_insert(wkey, cand);
+                        __hip_atomic_fetch_min(&wlane[wslot], lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    __syncthreads();
+                    const uint32_t owner =
+                        in_win ? __hip_atomic_load(&wlane[wslot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                               : lane;
                     // the accepted run cannot pass the first open lane still without a candidate
                     const uint64_t wait_mask = __ballot(open && !have);
                     const uint32_t lim = wait_mask ? (uint32_t)__builtin_ctzll(wait_mask) : width;
-                    bool dup_lane = false;
-                    for (uint32_t i = acc; i + 1u < lim; ++i) {
-                        const int32_t ci = __builtin_amdgcn_readlane(cand, (int)i);
-                        dup_lane |= lane > i && cand == ci;
-                    }
-                    const uint64_t bad_mask = __ballot(open && lane < lim && (in_set || dup_lane));
+                    const uint64_t bad_mask = __ballot(open && lane < lim && (in_set || owner < lane));
                     const uint32_t first = bad_mask ? (uint32_t)__builtin_ctzll(bad_mask) : lim;
                     if (open && lane < first) {
                         set_insert(hset, hbits, cand);
                         store_coherent(a.G + td + k, cand);
                     }
+                    if (in_win) {
+                        // every lane has read its owner above: clear the table for the next round
+                        __hip_atomic_store(&wkey[wslot], kEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_store(&wlane[wslot], (uint32_t)kWave, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
                     if (first > acc) progressed = true;
-                    acc = first;
-                    __syncthreads();
-                    if (mine && lane >= acc && have && set_contains(hset, hbits, cand)) {
+                    // Every open slot from `first` on whose candidate is in the set or
+                    // repeats an accepted slot's is rejected whatever the slots between
+                    // decide. The others stay checked: their candidates are neither in
+                    // the set nor among the values just added to it.
+                    if (open && lane >= first && have && (in_set || owner < first)) {
                         have = false;
                         need_draw = true;
-                        retried = true;
                         progressed = true;
+                        ++c_retries;
+                        kb = kb * 2 < srows ? kb * 2 : srows;
                         if (++r >= kMaxRetry) fail = kErrRetry;
                     }
-                    s_retries += (uint64_t)__popcll(__ballot(retried));
+                    checked = have;
+                    acc = first;
                     ++s_rounds;
                     if (__ballot(fail != 0) != 0) {
                         if (fail != 0) atomicCAS(&a.scratch->status, 0, fail);
@@ -952,11 +1051,11 @@ __global__ __launch_bounds__(64) void wide_generate_kernel(GenArgs a, uint32_t d
         }
     }
 
-    const uint64_t draws = wave_sum64(c_draws);
+    const uint64_t draws = wave_sum64(c_draws), retries = wave_sum64(c_retries);
     const uint64_t pmiss = wave_sum64(c_pmiss), polls = wave_sum64(c_polls), stalls = wave_sum64(c_stalls);
     if (lane == 0) {
         unsigned long long *st = a.scratch->stats;
-        const uint64_t v[10] = {s_nodes, s_nodes * d, draws, s_retries, pmiss, polls, stalls, s_claims, s_sleeps,
+        const uint64_t v[10] = {s_nodes, s_nodes * d, draws, retries, pmiss, polls, stalls, s_claims, s_sleeps,
                                 s_rounds};
         const int idx[10] = {kStatNodes,  kStatSlots,  kStatDraws,  kStatRetries, kStatPrefetchMisses,
                              kStatPolls,  kStatStalls, kStatClaims, kStatSleeps,  kStatPasses};
@@ -1354,9 +1453,8 @@ hipError_t launch_generate_wide(uint32_t d, const GenArgs &a, uint32_t blocks, u
 {
     // a.chunk nodes per claim: zero would never advance the claims
     if (d <= kNarrowMaxD || d > PPA_MAX_D || threads != kWideThreads || a.chunk == 0) return hipErrorInvalidValue;
-    const uint32_t hbits = wide_hash_bits(d);
-    hipLaunchKernelGGL(wide_generate_kernel, dim3(blocks), dim3(kWideThreads), (size_t)4u << hbits, stream, a, d,
-                       hbits, 1.0 / d);
+    hipLaunchKernelGGL(wide_generate_kernel, dim3(blocks), dim3(kWideThreads), wide_lds_bytes(d), stream, a, d,
+                       wide_hash_bits(d), wide_spec_rows(d), 1.0 / d);
     return hipGetLastError();
 }
 
@@ -1439,7 +1537,7 @@ hipError_t generate_occupancy(uint32_t d, uint32_t threads, int *out)
     default:
         if (d > PPA_MAX_D || threads != kWideThreads) return hipErrorInvalidValue;
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, wide_generate_kernel, (int)kWideThreads,
-                                                            (size_t)4u << wide_hash_bits(d));
+                                                            wide_lds_bytes(d));
     }
 }
 
