@@ -23,9 +23,9 @@ bool tuning_valid(const ppa_tuning &t)
     return true;
 }
 
-// d > kNarrowMaxD: one 64-thread workgroup per node in flight, as many as
-// the CUs hold (the hash set's LDS limits that at large d), but no more than
-// the graph has claims. A claim is wide_claim_nodes(d) consecutive nodes.
+// d > kNarrowMaxD: 64-thread workgroups of wide_geometry(d).streams windows
+// each, as many as the CUs hold (the LDS limits that at large d), but no more
+// than the graph has claims of wide_geometry(d).claim consecutive nodes.
 static int resolve_wide_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, LaunchShape *out)
 {
     const ppa_tuning &tu = ctx->tuning;
@@ -33,7 +33,7 @@ static int resolve_wide_shape(const ppa_ctx *ctx, uint32_t d, uint64_t work, Lau
     PPA_HIP(generate_occupancy(d, kWideThreads, &per_cu));
     if (per_cu < 1) per_cu = 1;
     if (tu.blocks_per_cu && (int)tu.blocks_per_cu < per_cu) per_cu = (int)tu.blocks_per_cu;
-    const uint32_t per_claim = wide_claim_nodes(d);
+    const uint32_t per_claim = wide_geometry(d).claim;
     const uint64_t claims = (work + per_claim - 1) / per_claim;
     uint64_t g = std::min<uint64_t>((uint64_t)per_cu * (uint64_t)ctx->num_cus, claims);
     if (tu.max_blocks && g > tu.max_blocks) g = tu.max_blocks;

@@ -774,30 +774,40 @@ void generate_kernel(GenArgs a)
 // would not fit, and a node's d copy reads alone keep a wavefront's 64 lanes
 // busy. The reference bounds d by its 48 KB shared-memory wait queue of 6,144
 // entries per block (cuPPA_RRP_1.cu:49-56) and its int16 slot index
-// (Common/DataTypes.h:22-27); here d goes up to PPA_MAX_D. A wavefront
-// claims a few consecutive nodes at a time (about 256 slots' worth, in
-// ascending order) and walks each node's slots in windows of 64, one slot per
-// lane. The node's accepted targets live in an LDS hash set (linear probing,
-// at most half full, cleared per node), so a duplicate check costs a few LDS
-// reads instead of a scan of up to d - 1 earlier slots. One round of a window:
+// (Common/DataTypes.h:22-27); here d goes up to PPA_MAX_D.
+//
+// A node's slots are cut into windows of 64, one slot per lane. A wavefront
+// runs S streams (8, or 4 above d = 256): a stream is one window of one node.
+// The wavefront claims a few consecutive nodes at a time (about one window
+// per stream, ascending) and hands their windows to idle streams in order,
+// so up to S windows -- of up to S nodes, or S windows of one large node --
+// are in flight at once: their draws and source reads overlap, and no
+// claimed node waits for the wavefront to finish the one before it. Each node
+// in flight has an LDS hash set of its accepted targets (linear probing, at
+// most half full), so a duplicate check costs a few LDS reads instead of a
+// scan of up to d - 1 earlier slots. Only a node's earliest unfinished window
+// (its "gate") accepts slots; the later ones draw and read ahead. One round:
 //   1. lanes whose slot needs an attempt draw it: a direct target is known at
 //      once, a copy draw reads its source slot straight into LDS (a plain read
 //      first -- slots are write-once, so it returns -1 or the final value --
 //      and agent-scope re-reads while the source is still empty). A slot
 //      rejected before draws its next kb attempts at once (kb doubling up to
-//      16 or 64 rows) and takes the first not already in the set: a node whose t
+//      4 or 16) and takes the first not already in the set: a node whose t
 //      candidates barely exceed its d slots (t close to d) has a coupon
 //      collector's tail of thousands of attempts in its last slots, and each
 //      of them would otherwise cost a memory round trip of its own;
-//   2. the first open lane (from `acc`, the window's accepted slots) whose
-//      candidate is missing, already in the set, or equal to an earlier open
-//      lane's candidate ends the accepted run: the lanes before it insert
-//      their candidates into the set and publish their slots at once;
-//   3. every open lane whose candidate is now in the set draws its next
-//      attempt: it is rejected whatever the slots between decide.
+//   2. in a gate window, the first open lane (from `acc`, the window's
+//      accepted slots) whose candidate is missing, already in the set, or
+//      equal to an earlier open lane's candidate ends the accepted run: the
+//      lanes before it insert their candidates into the set and publish their
+//      slots at once;
+//   3. every open lane whose candidate is in the set (in any window), or
+//      repeats an accepted one (in the gate window), draws its next attempt:
+//      it is rejected whatever the slots between decide.
 // The draws and the order of acceptance are the sequential model's, so the
-// graph is the same. The smallest unfinished node only waits on finished,
-// published slots, so the grid always makes progress.
+// graph is the same. The smallest unfinished node's gate window only waits on
+// finished, published slots and is always held by a stream (windows are
+// handed out and finished in order), so the grid always makes progress.
 
 __device__ __forceinline__ uint32_t set_slot(int32_t v, uint32_t hbits)
 {
@@ -826,19 +836,20 @@ __device__ __forceinline__ void set_insert(int32_t *set, uint32_t hbits, int32_t
     }
 }
 
-// Rows of attempts drawn at once (kb's cap): 64 for d >= 512, whose first
-// nodes have the longest retry tails (the last slot of node d + 1 takes about
-// d / 2 attempts), 16 below, where the LDS goes to occupancy instead.
-uint32_t wide_spec_rows(uint32_t d) { return d >= 512 ? 64u : 16u; }
-
-// The window table: for each candidate value of a round's open lanes, the
-// lowest lane holding it (128 keys, at most 64 in use, cleared per round).
+// The window table of a stream: for each candidate value of a round's open
+// lanes, the lowest lane holding it (128 keys, at most 64 in use, cleared per
+// round).
 constexpr uint32_t kWinBits = 7;
+static_assert(kWideWinWords == 2u << kWinBits, "keys and lowest lanes");
 
-size_t wide_lds_bytes(uint32_t d)
-{
-    return (((size_t)1 << wide_hash_bits(d)) + 2u * (1u << kWinBits) + (size_t)wide_spec_rows(d) * kWave) * 4u;
-}
+// Per-lane stream flags, packed with the attempt batch size kb (bits 0..7).
+constexpr uint32_t kKbMask = 0xffu;
+constexpr uint32_t kFNeed = 1u << 8;    // the slot's next kb attempts are to be drawn
+constexpr uint32_t kFPend = 1u << 9;    // waiting on the copy source src
+constexpr uint32_t kFHave = 1u << 10;   // cand holds the current attempt's target
+constexpr uint32_t kFChk = 1u << 11;    // cand is known not to be in the node's set
+constexpr uint32_t kFInSet = 1u << 12;  // this round: cand is in the set
+constexpr uint32_t kFInWin = 1u << 13;  // this round: cand is entered in the window table
 
 // The slot of v in the window table, inserting it if absent.
 __device__ __forceinline__ uint32_t win_insert(int32_t *keys, int32_t v)
@@ -853,201 +864,324 @@ __device__ __forceinline__ uint32_t win_insert(int32_t *keys, int32_t v)
     }
 }
 
-// One wavefront per workgroup. Dynamic LDS: the hash set (1 << hbits int32
-// words, at least 2d), the window table's keys and lowest lanes, then srows
-// rows of 64 words where attempt r + j of lane l lands in word l of row j.
+// One wavefront per workgroup, S streams. Dynamic LDS (int32 words): nslots
+// hash sets of 1 << hbits words (at least 2d each), S window tables (keys,
+// then lowest lanes), then S groups of srows rows of 64 words: attempt r + j
+// of lane l of stream s lands in word l of row j of group s, and a re-read of
+// its pending source in row 0.
+template <int S>
 __global__ __launch_bounds__(64) void wide_generate_kernel(GenArgs a, uint32_t d, uint32_t hbits, uint32_t srows,
-                                                           double inv_d)
+                                                           uint32_t prows, uint32_t nslots, double inv_d)
 {
     extern __shared__ int32_t wide_lds[];
-    int32_t *const hset = wide_lds;
-    int32_t *const wkey = hset + (1u << hbits);
-    uint32_t *const wlane = reinterpret_cast<uint32_t *>(wkey + (1u << kWinBits));
-    int32_t *const spec = wkey + 2u * (1u << kWinBits);
+    const uint32_t H = 1u << hbits;
+    int32_t *const sets = wide_lds;
+    int32_t *const wkey = sets + nslots * H;
+    uint32_t *const wlane = reinterpret_cast<uint32_t *>(wkey + S * (1u << kWinBits));
+    int32_t *const spec = wkey + S * kWideWinWords;
+    int32_t *const pool = spec + S * srows * kWave;  // prows more rows for the oldest node's gate
+    // row j of stream s: its own srows rows, then the pool's
+    auto row = [&](int s, uint32_t j) {
+        return j < srows ? spec + ((uint32_t)s * srows + j) * kWave : pool + (j - srows) * kWave;
+    };
+    const uint32_t kb_max = srows + prows;
     const uint32_t lane = lane_id();
-    const uint32_t hsize = 1u << hbits;
+    const uint32_t nwin = (d + kWave - 1) / kWave;
     const uint64_t start_clock = wall_clock64();
-    for (uint32_t i = lane; i < (1u << kWinBits); i += kWave) {
+    for (uint32_t i = lane; i < S * (1u << kWinBits); i += kWave) {
         wkey[i] = kEmpty;
         wlane[i] = kWave;
     }
-    const uint32_t per_claim = a.chunk;  // wide_claim_nodes(d), >= 1
+    __syncthreads();
+
+    // Wavefront-uniform hand-out state: claimed nodes [nt, ce), window nw of
+    // node nt next; cur_slot is node nt's hash set once its first window is out.
+    uint64_t nt = 0, ce = 0;
+    uint32_t nw = 0, cur_slot = 0, free_slots = (1u << nslots) - 1u;
+    bool claims_done = false;
+    // Wavefront-uniform stream state: its node (n < 2^31), and packed: in use
+    // (bit 0), hash set (bits 1..3), window (bits 4..11), accepted slots of
+    // the window (bits 12..18).
+    uint32_t st_t[S], st_x[S];
+    auto on = [&](int s) { return (st_x[s] & 1u) != 0; };
+    auto st_ns = [&](int s) { return (st_x[s] >> 1) & 7u; };
+    auto st_w = [&](int s) { return (st_x[s] >> 4) & 0xffu; };
+    auto st_acc = [&](int s) { return (st_x[s] >> 12) & 0x7fu; };
+    auto width_of = [&](int s) {
+        const uint32_t rest = d - st_w(s) * kWave;
+        return rest < (uint32_t)kWave ? rest : (uint32_t)kWave;
+    };
+    // Per-lane stream state: the slot's candidate, its current attempt (or the
+    // next one to draw), flags and kb, window-table slot, pending source.
+    int32_t cand[S];
+    uint32_t rr[S], fl[S], wslot[S];
+    uint64_t src[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        st_t[s] = st_x[s] = 0;
+        cand[s] = kEmpty;
+        rr[s] = fl[s] = wslot[s] = 0;
+        src[s] = 0;
+    }
     uint64_t s_nodes = 0, s_claims = 0, s_sleeps = 0, s_rounds = 0;
     uint64_t c_draws = 0, c_retries = 0;
     uint32_t c_pmiss = 0, c_polls = 0, c_stalls = 0;
     uint32_t backoff = 1;
     int fail = 0;
-    bool stop = false;
 
-    while (!stop) {
-        uint64_t base = 0;
-        if (lane == 0) base = atomicAdd(&a.scratch->claim, (unsigned long long)per_claim);
-        base = uniform64(base);
-        ++s_claims;
-        if (base >= a.n) break;
-        const uint64_t end = base + per_claim < a.n ? base + per_claim : a.n;
-        for (uint64_t t = base; t < end && !stop; ++t) {
-            for (uint32_t i = lane; i < hsize; i += kWave) hset[i] = kEmpty;
-            __syncthreads();
-            const uint64_t td = t * (uint64_t)d;
-            for (uint32_t w0 = 0; w0 < d && !stop; w0 += kWave) {
-                const uint32_t width = d - w0 < (uint32_t)kWave ? d - w0 : (uint32_t)kWave;
-                const bool mine = lane < width;
-                const uint32_t k = w0 + lane;
-                // r: the slot's current attempt (its candidate or pending
-                // source), or the next one to draw; kb: attempts the next draw takes
-                uint32_t r = 0, kb = 1;
-                // checked: cand is known not to be in the set as it stands
-                bool need_draw = mine, pending = false, have = false, checked = false;
-                uint64_t src = 0;
-                int32_t cand = kEmpty;
-                uint32_t acc = 0;  // wavefront-uniform: slots w0 .. w0+acc-1 are accepted
-                while (acc < width) {
-                    bool progressed = false;
-                    // ---- draws (attempt r + j into row j) and re-reads (row 0), all in flight at once ----
-                    for (uint32_t j = 0; j < srows; ++j) {
-                        const bool dj = need_draw && j < kb;
-                        if (__ballot(dj) == 0) break;
-                        if (dj) {
-                            const Attempt at = attempt(a.key0, a.key1, a.T, t, k, r + j, td);
-                            ++c_draws;
-                            if (at.direct)
-                                spec[j * kWave + lane] = (int32_t)div_small(at.pair, d, inv_d);
-                            else
-                                __builtin_amdgcn_global_load_lds(
-                                    (const __attribute__((address_space(1))) void *)(a.G + at.pair),
-                                    (__attribute__((address_space(3))) void *)(spec + j * kWave), 4, 0, 0);
-                        }
-                    }
-                    if (pending) {
-                        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(a.G + src),
-                                                         (__attribute__((address_space(3))) void *)spec, 4, 0, kSc1);
-                        ++c_polls;
-                    }
-                    __builtin_amdgcn_s_waitcnt(kWaitVm0);
-                    __syncthreads();
-                    if (need_draw) {
-                        // the first drawn attempt not already rejected by the set
-                        need_draw = false;
-                        progressed = true;
-                        uint32_t j = 0;
-                        bool done = false;
-                        while (!done && j < kb) {
-                            if (r + j >= kMaxRetry) {
-                                fail = kErrRetry;
-                                done = true;
-                                break;
-                            }
-                            const int32_t v = spec[j * kWave + lane];
-                            if (v <= kEmpty) {
-                                // its copy source is still empty: re-derive the source and wait on it
-                                ++c_pmiss;
-                                src = attempt(a.key0, a.key1, a.T, t, k, r + j, td).pair;
-                                pending = true;
-                                done = true;
-                            } else if (set_contains(hset, hbits, v)) {
-                                ++c_retries;  // it repeats an accepted slot: rejected
-                                ++j;
-                            } else {
-                                cand = v;
-                                have = true;
-                                checked = true;
-                                done = true;
-                            }
-                        }
-                        r += j;
-                        if (!done) {
-                            need_draw = true;
-                            kb = kb * 2 < srows ? kb * 2 : srows;
-                        }
-                    } else if (pending) {
-                        const int32_t v = spec[lane];
-                        if (v > kEmpty) {
-                            cand = v;
-                            have = true;
-                            checked = false;
-                            pending = false;
-                            progressed = true;
-                        } else {
-                            ++c_stalls;
-                        }
-                    }
-                    // ---- accept the run from acc up to the first slot that must wait or is rejected ----
-                    const bool open = mine && lane >= acc;
-                    const bool in_set = open && have && !checked && set_contains(hset, hbits, cand);
-                    // the lowest open lane holding each candidate value
-                    const bool in_win = open && have && !in_set;
-                    uint32_t wslot = 0;
-                    if (in_win) {
-                        wslot = win_insert(wkey, cand);
-                        __hip_atomic_fetch_min(&wlane[wslot], lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    }
-                    __syncthreads();
-                    const uint32_t owner =
-                        in_win ? __hip_atomic_load(&wlane[wslot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                               : lane;
-                    // the accepted run cannot pass the first open lane still without a candidate
-                    const uint64_t wait_mask = __ballot(open && !have);
-                    const uint32_t lim = wait_mask ? (uint32_t)__builtin_ctzll(wait_mask) : width;
-                    const uint64_t bad_mask = __ballot(open && lane < lim && (in_set || owner < lane));
-                    const uint32_t first = bad_mask ? (uint32_t)__builtin_ctzll(bad_mask) : lim;
-                    if (open && lane < first) {
-                        set_insert(hset, hbits, cand);
-                        store_coherent(a.G + td + k, cand);
-                    }
-                    if (in_win) {
-                        // every lane has read its owner above: clear the table for the next round
-                        __hip_atomic_store(&wkey[wslot], kEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        __hip_atomic_store(&wlane[wslot], (uint32_t)kWave, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                    }
-                    if (first > acc) progressed = true;
-                    // Every open slot from `first` on whose candidate is in the set or
-                    // repeats an accepted slot's is rejected whatever the slots between
-                    // decide. The others stay checked: their candidates are neither in
-                    // the set nor among the values just added to it.
-                    if (open && lane >= first && have && (in_set || owner < first)) {
-                        have = false;
-                        need_draw = true;
-                        progressed = true;
-                        ++c_retries;
-                        kb = kb * 2 < srows ? kb * 2 : srows;
-                        if (++r >= kMaxRetry) fail = kErrRetry;
-                    }
-                    checked = have;
-                    acc = first;
-                    ++s_rounds;
-                    if (__ballot(fail != 0) != 0) {
-                        if (fail != 0) atomicCAS(&a.scratch->status, 0, fail);
-                        stop = true;
-                        break;
-                    }
-                    if ((s_rounds & 63u) == 0) {
-                        int st = 0;
-                        if (lane == 0) {
-                            st = __hip_atomic_load(&a.scratch->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            if (st == 0 && wall_clock64() - start_clock > a.watchdog_ticks) {
-                                atomicCAS(&a.scratch->status, 0, kErrTimeout);
-                                st = kErrTimeout;
-                            }
-                        }
-                        if (__builtin_amdgcn_readfirstlane(st) != 0) {
-                            stop = true;
-                            break;
-                        }
-                    }
-                    if (__ballot(progressed) == 0) {
-                        // every open slot waits on another node's slot
-                        ++s_sleeps;
-                        for (uint32_t s = 0; s < backoff; ++s) __builtin_amdgcn_s_sleep(4);
-                        backoff = backoff * 2 < a.sleep_cap ? backoff * 2 : a.sleep_cap;
-                    } else {
-                        backoff = 1;
-                    }
+    while (true) {
+        bool progressed = false;
+        // ---- 1. idle streams take the next windows, in order ----
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (on(s)) continue;
+            if (nw == 0 && nt >= ce && !claims_done) {
+                uint64_t base = 0;
+                if (lane == 0) base = atomicAdd(&a.scratch->claim, (unsigned long long)a.chunk);
+                base = uniform64(base);
+                ++s_claims;
+                if (base >= a.n) {
+                    claims_done = true;
+                } else {
+                    nt = base;
+                    ce = base + a.chunk < a.n ? base + a.chunk : a.n;
                 }
             }
-            if (!stop) ++s_nodes;
-            __syncthreads();  // the set's last lookups before the next node clears it
+            // a new node needs a free hash set; the windows of node nt follow in order
+            if (nt < ce && (nw != 0 || free_slots != 0)) {
+                if (nw == 0) {
+                    cur_slot = (uint32_t)__builtin_ctz(free_slots);
+                    free_slots &= free_slots - 1u;
+                    int32_t *const set = sets + cur_slot * H;
+                    for (uint32_t i = lane; i < H; i += kWave) set[i] = kEmpty;
+                }
+                st_t[s] = (uint32_t)nt;
+                st_x[s] = 1u | cur_slot << 1 | nw << 4;
+                const uint32_t width = width_of(s);
+                cand[s] = kEmpty;
+                rr[s] = 0;
+                fl[s] = lane < width ? (kFNeed | 1u) : 1u;
+                progressed = true;
+                if (++nw == nwin) {
+                    nw = 0;
+                    ++nt;
+                }
+            }
+        }
+        bool any = false;
+#pragma unroll
+        for (int s = 0; s < S; ++s) any |= on(s);
+        if (!any && claims_done) break;
+
+        // ---- gates: a stream is its node's gate when no stream holds an
+        // earlier window of the same node (earlier windows were handed out
+        // first, so they are finished); the gate of the oldest node may use
+        // the pool rows ----
+        uint32_t gates = 0, pool_owner = S, pool_t = 0xffffffffu;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            bool g = on(s);
+#pragma unroll
+            for (int q = 0; q < S; ++q)
+                if (q != s && on(q) && st_t[q] == st_t[s] && st_w(q) < st_w(s)) g = false;
+            gates |= g ? 1u << s : 0u;
+            if (g && st_t[s] < pool_t) {
+                pool_owner = s;
+                pool_t = st_t[s];
+            }
+        }
+
+        // ---- 2. draws (attempt r + j into row j) and re-reads (row 0), all in flight at once ----
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (!on(s)) continue;
+            const uint64_t t = st_t[s], td = t * (uint64_t)d;
+            const uint32_t k = st_w(s) * kWave + lane;
+            const uint32_t cap = (uint32_t)s == pool_owner ? kb_max : srows;
+            uint32_t kb = fl[s] & kKbMask;
+            kb = kb < cap ? kb : cap;
+            fl[s] = (fl[s] & ~kKbMask) | kb;  // what is drawn now is what step 3 reads
+            const bool need = (fl[s] & kFNeed) != 0;
+            for (uint32_t j = 0; j < cap; ++j) {
+                const bool dj = need && j < kb;
+                if (__ballot(dj) == 0) break;
+                if (dj) {
+                    const Attempt at = attempt(a.key0, a.key1, a.T, t, k, rr[s] + j, td);
+                    ++c_draws;
+                    int32_t *const rj = row(s, j);
+                    if (at.direct)
+                        rj[lane] = (int32_t)div_small(at.pair, d, inv_d);
+                    else
+                        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(a.G + at.pair),
+                                                         (__attribute__((address_space(3))) void *)rj, 4, 0, 0);
+                }
+            }
+            if (fl[s] & kFPend) {
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(a.G + src[s]),
+                                                 (__attribute__((address_space(3))) void *)row(s, 0), 4, 0, kSc1);
+                ++c_polls;
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+        __syncthreads();
+
+        // ---- 3. the first drawn attempt not already rejected by the set; resolved re-reads ----
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (!on(s)) continue;
+            const int32_t *const set = sets + st_ns(s) * H;
+            uint32_t f = fl[s];
+            if (f & kFNeed) {
+                f &= ~kFNeed;
+                progressed = true;
+                const uint32_t kb = f & kKbMask;
+                uint32_t j = 0;
+                bool done = false;
+                while (!done && j < kb) {
+                    if (rr[s] + j >= kMaxRetry) {
+                        fail = kErrRetry;
+                        done = true;
+                        break;
+                    }
+                    const int32_t v = row(s, j)[lane];
+                    if (v <= kEmpty) {
+                        // its copy source is still empty: re-derive the source and wait on it
+                        ++c_pmiss;
+                        src[s] = attempt(a.key0, a.key1, a.T, st_t[s], st_w(s) * kWave + lane, rr[s] + j,
+                                         (uint64_t)st_t[s] * d)
+                                     .pair;
+                        f |= kFPend;
+                        done = true;
+                    } else if (set_contains(set, hbits, v)) {
+                        ++c_retries;  // it repeats an accepted slot: rejected
+                        ++j;
+                    } else {
+                        cand[s] = v;
+                        f |= kFHave | kFChk;
+                        done = true;
+                    }
+                }
+                rr[s] += j;
+                if (!done) f = (f & ~kKbMask) | kFNeed | (kb * 2 < kb_max ? kb * 2 : kb_max);
+            } else if (f & kFPend) {
+                const int32_t v = row(s, 0)[lane];
+                if (v > kEmpty) {
+                    cand[s] = v;
+                    f = (f & ~(kFPend | kFChk)) | kFHave;
+                    progressed = true;
+                } else {
+                    ++c_stalls;
+                }
+            }
+            fl[s] = f;
+        }
+
+        // ---- 4. set lookups, window tables (gates as found above) ----
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (!on(s)) continue;
+            const bool gate = (gates >> s) & 1u;
+            const int32_t *const set = sets + st_ns(s) * H;
+            const bool open = lane < width_of(s) && lane >= st_acc(s);
+            uint32_t f = fl[s] & ~(kFInSet | kFInWin);
+            const bool have = (f & kFHave) != 0;
+            // a window behind the gate looks every candidate up again: the
+            // gate may have added to the set since
+            const bool chk = gate && (f & kFChk) != 0;
+            if (open && have && !chk && set_contains(set, hbits, cand[s])) f |= kFInSet;
+            if (gate && open && have && !(f & kFInSet)) {
+                f |= kFInWin;
+                wslot[s] = win_insert(wkey + (uint32_t)s * (1u << kWinBits), cand[s]);
+                __hip_atomic_fetch_min(&wlane[(uint32_t)s * (1u << kWinBits) + wslot[s]], lane, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            fl[s] = f;
+        }
+        __syncthreads();
+
+        // ---- 5. accept in order in the gates; reject what is decided ----
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (!on(s)) continue;
+            const bool gate = (gates >> s) & 1u;
+            int32_t *const set = sets + st_ns(s) * H;
+            const uint32_t width = width_of(s);
+            const uint32_t acc = st_acc(s);
+            const bool open = lane < width && lane >= acc;
+            uint32_t f = fl[s];
+            const bool have = (f & kFHave) != 0, in_set = (f & kFInSet) != 0, in_win = (f & kFInWin) != 0;
+            bool reject;
+            uint32_t first = acc;
+            if (gate) {
+                const uint32_t ws = (uint32_t)s * (1u << kWinBits) + wslot[s];
+                const uint32_t owner =
+                    in_win ? __hip_atomic_load(&wlane[ws], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : lane;
+                // the accepted run cannot pass the first open lane still without a candidate
+                const uint64_t wait_mask = __ballot(open && !have);
+                const uint32_t lim = wait_mask ? (uint32_t)__builtin_ctzll(wait_mask) : width;
+                const uint64_t bad_mask = __ballot(open && lane < lim && (in_set || owner < lane));
+                first = bad_mask ? (uint32_t)__builtin_ctzll(bad_mask) : lim;
+                if (open && lane < first) {
+                    set_insert(set, hbits, cand[s]);
+                    store_coherent(a.G + (uint64_t)st_t[s] * d + st_w(s) * kWave + lane, cand[s]);
+                }
+                if (in_win) {
+                    // every lane has read its owner above: clear the table for the next round
+                    __hip_atomic_store(&wkey[ws], kEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_store(&wlane[ws], (uint32_t)kWave, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                if (first > acc) progressed = true;
+                // Every open slot from `first` on whose candidate is in the set
+                // or repeats an accepted slot's is rejected whatever the slots
+                // between decide; the others stay checked: their candidates are
+                // neither in the set nor among the values just added to it.
+                reject = open && lane >= first && have && (in_set || owner < first);
+                st_x[s] = (st_x[s] & ~(0x7fu << 12)) | first << 12;
+            } else {
+                reject = open && have && in_set;
+            }
+            if (reject) {
+                const uint32_t kb = f & kKbMask;
+                f = (f & ~(kFHave | kFChk | kKbMask)) | kFNeed | (kb * 2 < kb_max ? kb * 2 : kb_max);
+                ++c_retries;
+                progressed = true;
+                if (++rr[s] >= kMaxRetry) fail = kErrRetry;
+            }
+            f = gate && (f & kFHave) ? (f | kFChk) : (f & ~kFChk);
+            fl[s] = f;
+            if (gate && first == width) {
+                // window done: the stream is free; the node is done with its last window
+                if (st_w(s) == nwin - 1u) {
+                    ++s_nodes;
+                    free_slots |= 1u << st_ns(s);
+                }
+                st_x[s] = 0;
+            }
+        }
+
+        ++s_rounds;
+        if (__ballot(fail != 0) != 0) {
+            if (fail != 0) atomicCAS(&a.scratch->status, 0, fail);
+            break;
+        }
+        if ((s_rounds & 63u) == 0) {
+            int st = 0;
+            if (lane == 0) {
+                st = __hip_atomic_load(&a.scratch->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (st == 0 && wall_clock64() - start_clock > a.watchdog_ticks) {
+                    atomicCAS(&a.scratch->status, 0, kErrTimeout);
+                    st = kErrTimeout;
+                }
+            }
+            if (__builtin_amdgcn_readfirstlane(st) != 0) break;
+        }
+        if (__ballot(progressed) == 0) {
+            // every open slot of every stream waits on another node's slot
+            ++s_sleeps;
+            for (uint32_t i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(4);
+            backoff = backoff * 2 < a.sleep_cap ? backoff * 2 : a.sleep_cap;
+        } else {
+            backoff = 1;
         }
     }
 
@@ -1453,9 +1587,33 @@ hipError_t launch_generate_wide(uint32_t d, const GenArgs &a, uint32_t blocks, u
 {
     // a.chunk nodes per claim: zero would never advance the claims
     if (d <= kNarrowMaxD || d > PPA_MAX_D || threads != kWideThreads || a.chunk == 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(wide_generate_kernel, dim3(blocks), dim3(kWideThreads), wide_lds_bytes(d), stream, a, d,
-                       wide_hash_bits(d), wide_spec_rows(d), 1.0 / d);
+    const WideGeometry g = wide_geometry(d);
+    // kb is kept in 8 bits; a stream holds at most 64 lanes' windows of 64 slots
+    if (g.srows + g.prows > 255 || g.nslots > 8 || g.nslots < 1) return hipErrorInvalidValue;
+#define PPA_WIDE(SS)                                                                                            \
+    case SS:                                                                                                    \
+        hipLaunchKernelGGL(wide_generate_kernel<SS>, dim3(blocks), dim3(kWideThreads), wide_lds_bytes(g), stream, \
+                           a, d, g.hbits, g.srows, g.prows, g.nslots, 1.0 / d);                                 \
+        break;
+    switch (g.streams) {
+        PPA_WIDE(1) PPA_WIDE(2) PPA_WIDE(4) PPA_WIDE(8)
+    default: return hipErrorInvalidValue;
+    }
+#undef PPA_WIDE
     return hipGetLastError();
+}
+
+hipError_t wide_occupancy(uint32_t d, int *out)
+{
+    const WideGeometry g = wide_geometry(d);
+    switch (g.streams) {
+#define PPA_WIDE(SS) \
+    case SS: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, wide_generate_kernel<SS>, (int)kWideThreads, \
+                                                                 wide_lds_bytes(g));
+        PPA_WIDE(1) PPA_WIDE(2) PPA_WIDE(4) PPA_WIDE(8)
+#undef PPA_WIDE
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
@@ -1536,8 +1694,7 @@ hipError_t generate_occupancy(uint32_t d, uint32_t threads, int *out)
 #undef PPA_CASE
     default:
         if (d > PPA_MAX_D || threads != kWideThreads) return hipErrorInvalidValue;
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, wide_generate_kernel, (int)kWideThreads,
-                                                            wide_lds_bytes(d));
+        return wide_occupancy(d, out);
     }
 }
 

@@ -67,7 +67,7 @@ struct GenArgs {
     uint64_t nseg;
     // common
     uint32_t chunk;           // work claimed per wavefront claim (multiple of 64; d > kNarrowMaxD:
-                              // nodes, wide_claim_nodes(d))
+                              // nodes, wide_geometry(d).claim)
     uint32_t early_chunk;     // claim size below early_nodes (single graph)
     uint64_t early_nodes;     // below this node index every slot is published as accepted
     uint64_t watchdog_ticks;  // wall-clock ticks a wavefront may run
@@ -182,20 +182,67 @@ hipError_t launch_degree_histogram(const uint32_t *deg, uint64_t n, uint64_t *hi
 
 constexpr uint32_t kBatchDMax = 16;  // d of 0 selects the batch kernel in generate_occupancy
 
-// Out-degrees above kNarrowMaxD run the wide generator: one wavefront (a
-// 64-thread workgroup) per node, whose accepted targets sit in an LDS hash set
-// of 1 << wide_hash_bits(d) int32 words, the smallest power of two >= 2d (at
-// most half full; 64 KB at d = 8192).
+// Out-degrees above kNarrowMaxD run the wide generator: 64-thread workgroups
+// (one wavefront) whose streams each work on one 64-slot window of a node;
+// each node in flight keeps its accepted targets in an LDS hash set of
+// 1 << hbits int32 words, the smallest power of two >= 2d (at most half full;
+// 64 KB at d = 8192).
 constexpr uint32_t kNarrowMaxD = 32;
 constexpr uint32_t kWideThreads = 64;
+constexpr uint32_t kWideWinWords = 256;  // a stream's window table: 128 keys, 128 lowest lanes
 inline uint32_t wide_hash_bits(uint32_t d)
 {
     uint32_t h = 1;
     while ((1u << h) < 2u * d) ++h;
     return h;
 }
-// Nodes a wavefront of the wide generator claims at once (GenArgs::chunk):
-// about 256 slots' worth, so that claims cost one atomic per 4+ rounds.
-inline uint32_t wide_claim_nodes(uint32_t d) { return d >= 256 ? 1u : (256u + d - 1u) / d; }
+struct WideGeometry {
+    uint32_t streams;  // windows in flight per wavefront: 8, 4 or 1
+    uint32_t srows;    // attempts a rejected slot draws at once, at most: rows of 64 words per
+                       // stream (64 with one stream: the first nodes' retry tails grow with d,
+                       // the last slot of node d + 1 takes about d / 2 attempts)
+    uint32_t prows;    // more rows for the gate window of the wavefront's oldest node, whose
+                       // slots are on the critical path (kb up to srows + prows there)
+    uint32_t nslots;   // nodes in flight per wavefront (a hash set each): one more than a claim,
+                       // up to 8, within 32 KB of sets
+    uint32_t hbits;
+    uint32_t claim;    // nodes per claim (GenArgs::chunk)
+};
+// Streams and claim sizes by out-degree, as measured on n*d = 2^30 .. 2^31
+// slots at p = 0.2: more streams overlap more windows, but every window
+// behind a node's gate re-reads its still-empty sources while it waits, which
+// past d = 192 costs more than the overlap gains (d = 256: 110 ms with one
+// stream, 124 with 4, 172 with 8; d = 1024: 264 / 353 / 588 / 1102 ms with
+// 1 / 2 / 4 / 8). Claims cost one atomic on one counter, which sustains about
+// 7e7 a second: at d = 33, 16 nodes a claim ran in 59 ms, 8 in 68, 2 in 136;
+// larger claims queue nodes behind each other (d = 64: 49 ms at 8, 92 at 32).
+inline WideGeometry wide_geometry(uint32_t d)
+{
+    WideGeometry g;
+    g.hbits = wide_hash_bits(d);
+    if (d > 192) {
+        // one window at a time, one node per wavefront: 64 rows for any slot
+        g.streams = 1u;
+        g.srows = 64u;
+        g.prows = 0u;
+        g.claim = 1u;
+        g.nslots = 1u;
+        return g;
+    }
+    g.streams = d <= 96 ? 8u : 4u;
+    g.claim = d <= 40 ? 16u : (d <= 64 ? 8u : (d <= 128 ? 4u : 2u));
+    g.srows = 4u;
+    g.prows = d > 64 ? 12u : 0u;
+    const uint32_t by_lds = 8192u >> g.hbits;  // hbits <= 9 here
+    const uint32_t want = g.claim + 1u < 8u ? g.claim + 1u : 8u;
+    g.nslots = want < by_lds ? want : by_lds;
+    return g;
+}
+inline size_t wide_lds_bytes(const WideGeometry &g)
+{
+    return (((size_t)g.nslots << g.hbits) + (size_t)g.streams * kWideWinWords +
+            ((size_t)g.streams * g.srows + g.prows) * 64u) *
+           4u;
+}
 
 }  // namespace ppa
