@@ -457,6 +457,45 @@ def run_latency(args):
                       "data": "synthetic (keyed Philox draws)", "graphs": out}))
 
 
+# Out-degrees above 32 (the wavefront-stream generator), at 2^30 .. 2^31
+# slots, beside d = 32 on the lane-per-node generator.
+WIDE_GRAPHS = [(1 << 25, 32, 0.2), (1 << 25, 33, 0.2), (1 << 24, 64, 0.2), (1 << 23, 128, 0.2),
+               (1 << 22, 256, 0.2), (1 << 20, 1024, 0.2), (1 << 17, 8192, 0.5)]
+
+
+def run_wide(args):
+    """Wall time and kernel time of one ppa_generate call per graph of
+    WIDE_GRAPHS (inputs resident, median of `steps` calls after `warmup`)."""
+    import statistics
+    import torch
+    import cuppa
+    ctx = cuppa.Context(0)
+    out = []
+    for n, d, p in WIDE_GRAPHS:
+        desc = cuppa.make_desc(n, d, p, 7)
+        targets = torch.empty(n * d, dtype=torch.int32, device="cuda")
+        ms, kernel = [], []
+        for i in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.generate_into(desc, targets, None)
+            if i >= args.warmup:
+                ms.append((time.perf_counter() - t0) * 1e3)
+                kernel.append(ctx.stats()["kernel_ms"])
+        st = ctx.stats()
+        call = statistics.median(ms)
+        out.append({"n": n, "d": d, "p": p, "call_ms": call, "kernel_ms": statistics.median(kernel),
+                    "edges_per_s": n * d / (call * 1e-3), "grid_blocks": st["grid_blocks"],
+                    "retries_per_node": st["retries"] / max(1, st["nodes"]),
+                    "rounds_per_node": st["passes"] / max(1, st["nodes"])})
+        print(f"bench: wide n={n} d={d} p={p}: {call:.2f} ms", file=sys.stderr, flush=True)
+        del targets
+        torch.cuda.empty_cache()
+    print(json.dumps({"metric": "edges/s of single graphs by out-degree (d > 32: wavefront streams)",
+                      "unit": "edges/s", "higher_is_better": True, "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "data": "synthetic (keyed Philox draws)", "graphs": out}))
+
+
 def degree_and_csr(ctx, n, d, edges, targets, row_ptr) -> dict:
     """The degree pass and the in-edge CSR of the generated graph: the first
     call of each also allocates the context's work memory; the second is
@@ -528,10 +567,10 @@ def main():
                          "and the reference's serial BA on a prefix")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB three times (counter calibration) and exit")
-    ap.add_argument("--mode", default="graph", choices=("graph", "batch", "serve", "latency", "probe"),
+    ap.add_argument("--mode", default="graph", choices=("graph", "batch", "serve", "latency", "probe", "wide"),
                     help="graph: the single-graph metric with the batch and serve objects (default); "
                          "batch / serve: one of those objects alone; latency: single-call wall times of "
-                         "small and mid-size graphs")
+                         "small and mid-size graphs; wide: single graphs of out-degree 32 .. 8192")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-runs", type=int, default=3, help="timed passes of each side (median, >= 3)")
     ap.add_argument("--batch-autotune", action="store_true",
@@ -581,6 +620,8 @@ def main():
         return run_latency(args)
     if args.mode == "probe":
         return run_probe(args)
+    if args.mode == "wide":
+        return run_wide(args)
 
     n, d, p, seed = WORKLOADS[args.workload]
     ctx = cuppa.Context(0)
