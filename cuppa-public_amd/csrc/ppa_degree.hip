@@ -1,4 +1,4 @@
-// ppa_degree.hip -- partitioned degree pass for gfx950.
+// ppa_degree.hip -- partitioned degree pass and in-edge CSR for gfx950.
 //
 // Replaces the host loops of cuPPA::printDegreeDistribution (cuPPA.cu:115-160)
 // and PAHelper::outputSCMDegreeDistribution (PAHelper.hpp:15-67):
@@ -6,19 +6,25 @@
 //
 // One device-scope atomic per slot is bound by the memory-side atomic rate
 // (a few 1e9 per second on this part), so the slots are instead partitioned
-// by target into "leaves" of 2^14 consecutive nodes and each leaf is counted
-// in LDS (64 KB of counters per workgroup):
-//   * level 1 partitions all slots by v >> 22 (up to 512 bins), level 2 each
-//     level-1 bucket by bits 14..21 (256 bins); graphs of at most 2^22 nodes
-//     need one level (v >> 14), and of at most 2^14 nodes none;
-//   * a partition pass counts bins per workgroup (LDS histogram), scans the
-//     counts in (segment, bin, workgroup) order, and scatters tile by tile:
-//     each 4096-slot tile is ordered by bin in LDS first, so each bin's run
-//     leaves the workgroup as contiguous stores;
+// by target into "leaves" of 2^15 consecutive nodes and each leaf is counted
+// in LDS (128 KB of counters per workgroup):
+//   * graphs of more than 2^23 nodes take two partition levels: level 1 by
+//     bucket v >> 23 (at most 256 buckets), level 2 each bucket by bits
+//     15..22 (256 leaves per bucket); graphs of at most 2^23 nodes one level
+//     (v >> 15), and of at most 2^15 nodes none;
+//   * one count pass reads the slots once and counts, per workgroup, both
+//     levels at once: the level-1 buckets and the (bucket, leaf) matrix, 128
+//     buckets x 256 leaves of LDS counters per launch (one launch up to 2^30
+//     nodes, two above). Scanned in (bucket, leaf, workgroup) order, the
+//     matrix places level 2 without a second count pass: level-2 workgroup j
+//     moves exactly the runs that level-1 workgroup j left in the buckets;
+//   * a scatter orders each 8192-slot tile by bin in LDS before storing it,
+//     so each bin's run leaves the workgroup as contiguous stores; the last
+//     level keeps only a key's low 15 bits (16-bit keys, half the bytes);
 //   * a leaf of at most kPiece slots is counted by one workgroup, which
-//     stores its 2^14 degrees with plain coalesced stores; larger leaves (the
-//     hubs of a preferential-attachment graph) are split into pieces whose
-//     LDS counts are merged with one atomic per node and piece.
+//     stores its 2^15 degrees with coalesced stores; larger leaves (the hubs
+//     of a preferential-attachment graph) are split into pieces whose LDS
+//     counts are merged with one atomic per node and piece.
 // Slot values outside [0, n) (an incomplete graph) are skipped everywhere.
 #include "ppa_degree.hpp"
 #include "ppa_device.hpp"
@@ -31,56 +37,27 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kPerThread = 16;
 constexpr int kTile = kThreads * kPerThread;  // slots a count workgroup reads at once
-constexpr int kScatterThreads = 512;           // scatter workgroups order 8192 slots at once
 constexpr uint32_t kMaxBins = 512;
-constexpr uint32_t kLeafBits = 14;
-constexpr uint32_t kLeafNodes = 1u << kLeafBits;
-constexpr uint32_t kLevel1Shift = 22;
-constexpr uint32_t kLevel2Bins = 256;
-constexpr uint64_t kPiece = 1ull << 20;       // slots of one leaf-count work item
-constexpr uint32_t kScanChunk = 4096;         // entries per scan workgroup
+constexpr uint32_t kScanChunk = 4096;         // entries per scan step of a workgroup
 constexpr uint32_t kScanParts = 256;          // scan workgroups at most (one top-level pass)
 constexpr uint32_t kMaxBlocks = 2048;         // partition workgroups at most
 
-struct Geometry {
-    uint64_t first = 0, slots = 0;  // generated slots [first, first + slots) of G
-    uint32_t levels = 0;            // partition passes: 0, 1 or 2
-    uint32_t bins1 = 0;             // bins of the first pass
-    uint32_t leaves = 1;            // leaf entries (Lcap)
-    uint32_t blocks1 = 0, blocks2 = 0;
-    uint64_t counts = 0;            // entries of the counts array
-};
-
-Geometry geometry(uint64_t n, uint32_t d, uint32_t num_cus)
-{
-    Geometry g;
-    g.first = (uint64_t)(d + 1) * d;
-    g.slots = n > (uint64_t)d + 1 ? (n - d - 1) * (uint64_t)d : 0;
-    const uint64_t leaves = (n + kLeafNodes - 1) >> kLeafBits;
-    // The count and scatter kernels of a pass share the workgroup ranges. 6
-    // per CU: the 512-thread scatter (4 resident per CU) then runs a second,
-    // partly empty round, yet 4 per CU (one round) measured the same 71.8 ms
-    // at n = 2^30, d = 8: the scatter is bound by memory, not by residency.
-    uint64_t b = std::min<uint64_t>((uint64_t)num_cus * 6, kMaxBlocks);
-    b = std::max<uint64_t>(1, std::min<uint64_t>(b, (g.slots + 65535) / 65536));
-    g.blocks1 = (uint32_t)b;
-    if (leaves <= 1) {
-        g.levels = 0;
-        g.leaves = 1;
-    } else if (leaves <= kLevel2Bins) {
-        g.levels = 1;
-        g.bins1 = (uint32_t)leaves;
-        g.leaves = g.bins1;
-        g.counts = (uint64_t)g.bins1 * g.blocks1 + 1;
-    } else {
-        g.levels = 2;
-        g.bins1 = (uint32_t)((n + (1ull << kLevel1Shift) - 1) >> kLevel1Shift);
-        g.leaves = g.bins1 * kLevel2Bins;
-        g.blocks2 = std::max<uint32_t>(g.blocks1, 2 * g.bins1 + 1);
-        g.counts = std::max<uint64_t>((uint64_t)g.bins1 * g.blocks1, (uint64_t)kLevel2Bins * g.blocks2) + 1;
-    }
-    return g;
-}
+// degree pass
+constexpr uint32_t kLeafBits = 15;            // a leaf: 2^15 nodes, 128 KB of LDS counters
+constexpr uint32_t kLeafNodes = 1u << kLeafBits;
+constexpr uint32_t kSubBits = 8;              // leaves per level-1 bucket of a two-level graph
+constexpr uint32_t kSubBins = 1u << kSubBits;
+constexpr uint32_t kBucketShift = kLeafBits + kSubBits;  // two-level bucket: v >> 23
+constexpr uint32_t kMaxBuckets = 256;         // n <= 2^31
+constexpr uint32_t kMatBuckets = 128;         // buckets one matrix count covers (128 KB of LDS)
+constexpr int kCountThreads = 1024;           // count workgroups: one per CU by LDS, 16 waves
+constexpr int kPartThreads = 512;             // scatter workgroups: 8192-slot tiles
+constexpr int kPartPer = 16;                  // slots per scatter thread per tile
+constexpr int kPartTile = kPartThreads * kPartPer;
+constexpr uint32_t kPartPerCU = 4;            // partition workgroups per CU
+constexpr int kLeafThreads = 1024;
+constexpr uint64_t kPiece = 1ull << 20;       // slots of one leaf-count work item
+constexpr int kPlanThreads = 1024;
 
 // One partition pass: segments of the input, each split into bins by
 // (v >> shift) & mask over its own workgroups.
@@ -156,69 +133,26 @@ __device__ uint64_t block_exclusive_scan(uint64_t v, uint64_t *total, uint64_t *
     return before + x - v;
 }
 
-__global__ void fill_u64_kernel(uint64_t *p, uint64_t len, uint64_t value)
-{
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride) p[i] = value;
-}
-
 __global__ void fill_deg_kernel(uint32_t *deg, uint64_t n, uint32_t d)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) deg[v] = d;
 }
 
-// A single segment over all slots, or a single leaf holding them.
-__global__ void plan_whole_kernel(uint32_t *seg_b0, uint64_t *seg_start, uint32_t blocks, uint64_t slots,
-                                  uint64_t *leaf_size)
+// A single segment over all slots.
+__global__ void plan_whole_kernel(uint32_t *seg_b0, uint64_t *seg_start, uint32_t blocks, uint64_t slots)
 {
     if (threadIdx.x == 0) {
         seg_b0[0] = 0;
         seg_b0[1] = blocks;
         seg_start[0] = 0;
         seg_start[1] = slots;
-        if (leaf_size) leaf_size[0] = slots;
     }
 }
 
-// Level-2 segments = level-1 buckets; workgroups in proportion to their size.
-__global__ __launch_bounds__(kMaxBins) void plan_level2_kernel(const uint64_t *offsets1, uint32_t blocks1,
-                                                               uint32_t nseg, uint32_t blocks2, uint32_t *seg_b0,
-                                                               uint64_t *seg_start)
-{
-    __shared__ uint64_t start[kMaxBins + 1];
-    __shared__ uint64_t red[kMaxBins / 64];
-    const uint32_t s = threadIdx.x;
-    // bucket s begins at the scanned count of (bin s, workgroup 0); the
-    // entry after the last bucket holds the total
-    if (s < nseg) start[s] = offsets1[(uint64_t)s * blocks1];
-    if (s == 0) start[nseg] = offsets1[(uint64_t)nseg * blocks1];
-    __syncthreads();
-    const uint64_t total = start[nseg];
-    const uint64_t per = total ? (total + (blocks2 - nseg) - 1) / (blocks2 - nseg) : 1;
-    const uint64_t size = s < nseg ? start[s + 1] - start[s] : 0;
-    const uint64_t nb = size ? (size + per - 1) / per : 0;  // sum <= (blocks2 - nseg) + nseg
-    uint64_t used = 0;
-    const uint64_t b0 = block_exclusive_scan<kMaxBins>(nb, &used, red);
-    if (s < nseg) {
-        seg_b0[s] = (uint32_t)b0;
-        seg_start[s] = start[s];
-    }
-    if (s == 0) {
-        seg_b0[nseg] = (uint32_t)used;
-        seg_start[nseg] = total;
-    }
-}
-
-// Bin 0 of a partition holds the smallest node ids, which in a
-// preferential-attachment graph are named by far the most slots (a third of
-// them at the first level): its updates are counted per lane, or ranked per
-// wavefront, instead of contending for one LDS word.
-
-// counts[(segment, bin, workgroup)] = slots of the workgroup's range in bin;
-// with leaf_size, also the slots of every (segment, bin) leaf.
+// counts[(segment, bin, workgroup)] = slots of the workgroup's range in bin.
 __global__ __launch_bounds__(kThreads) void part_count_kernel(const int32_t *in, PartPlan pl, uint64_t n,
-                                                              uint64_t *counts, uint64_t *leaf_size)
+                                                              uint64_t *counts)
 {
     __shared__ uint32_t hist[kMaxBins];
     if (blockIdx.x >= pl.seg_b0[pl.nseg]) return;
@@ -255,24 +189,22 @@ __global__ __launch_bounds__(kThreads) void part_count_kernel(const int32_t *in,
     hot = wave_sum(hot);
     if ((threadIdx.x & 63u) == 0 && hot) atomicAdd(&hist[0], hot);
     __syncthreads();
-    for (uint32_t bin = threadIdx.x; bin < pl.nbins; bin += kThreads) {
-        const uint32_t c = hist[bin];
-        counts[r.count_base + (uint64_t)bin * r.nb + r.rank] = c;
-        if (leaf_size && c)
-            atomicAdd((unsigned long long *)&leaf_size[(uint64_t)r.seg * pl.nbins + bin], (unsigned long long)c);
-    }
+    for (uint32_t bin = threadIdx.x; bin < pl.nbins; bin += kThreads)
+        counts[r.count_base + (uint64_t)bin * r.nb + r.rank] = hist[bin];
 }
 
 // Exclusive scan of counts[0, len) into offsets[0, len], offsets[len] = total:
-// workgroup sums, one top-level scan, then per-chunk scans.
-__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const uint64_t *counts, uint64_t len,
+// workgroup sums over chunks of `chunk` entries, one top-level scan, then
+// each workgroup scans its chunk kScanChunk entries at a time.
+template <typename C>
+__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const C *counts, uint64_t len, uint64_t chunk,
                                                                uint64_t *partials)
 {
     __shared__ uint64_t red[kThreads / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+    const uint64_t base = (uint64_t)blockIdx.x * chunk;
+    const uint64_t end = base + chunk < len ? base + chunk : len;
     uint64_t s = 0;
-    for (uint32_t i = threadIdx.x; i < kScanChunk; i += kThreads)
-        if (base + i < len) s += counts[base + i];
+    for (uint64_t i = base + threadIdx.x; i < end; i += kThreads) s += counts[i];
     uint64_t total = 0;
     (void)block_exclusive_scan<kThreads>(s, &total, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
@@ -289,64 +221,209 @@ __global__ __launch_bounds__(kScanParts) void scan_top_kernel(uint64_t *partials
     if (threadIdx.x == 0) *total_out = total;
 }
 
-__global__ __launch_bounds__(kThreads) void scan_apply_kernel(const uint64_t *counts, uint64_t len,
+template <typename C>
+__global__ __launch_bounds__(kThreads) void scan_apply_kernel(const C *counts, uint64_t len, uint64_t chunk,
                                                               const uint64_t *partials, uint64_t *offsets)
 {
     __shared__ uint64_t red[kThreads / 64];
     constexpr uint32_t kPer = kScanChunk / kThreads;  // consecutive entries per thread
-    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kPer;
-    uint64_t v[kPer];
-    uint64_t s = 0;
+    const uint64_t start = (uint64_t)blockIdx.x * chunk;
+    const uint64_t end = start + chunk < len ? start + chunk : len;
+    uint64_t carry = partials[blockIdx.x];
+    for (uint64_t step = start; step < end; step += kScanChunk) {
+        const uint64_t base = step + (uint64_t)threadIdx.x * kPer;
+        uint64_t v[kPer];
+        uint64_t s = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < kPer; ++j) {
-        v[j] = base + j < len ? counts[base + j] : 0;
-        s += v[j];
-    }
-    uint64_t total = 0;
-    uint64_t run = block_exclusive_scan<kThreads>(s, &total, red) + partials[blockIdx.x];
+        for (uint32_t j = 0; j < kPer; ++j) {
+            v[j] = base + j < end ? (uint64_t)counts[base + j] : 0;
+            s += v[j];
+        }
+        uint64_t total = 0;
+        uint64_t run = block_exclusive_scan<kThreads>(s, &total, red) + carry;
 #pragma unroll
-    for (uint32_t j = 0; j < kPer; ++j) {
-        if (base + j < len) offsets[base + j] = run;
-        run += v[j];
+        for (uint32_t j = 0; j < kPer; ++j) {
+            if (base + j < end) offsets[base + j] = run;
+            run += v[j];
+        }
+        carry += total;
     }
 }
 
-// Moves every valid slot of the workgroup's range to its bin's run of the
-// output; tile by tile, slots are grouped by bin in LDS before the stores.
-// TH threads place TH * kPerThread slots per tile: at 512 threads a bin's run
-// in a tile averages 32 slots at 256 bins, so most of a tile's stores fill
-// whole 64-byte lines.
-template <int TH>
-__global__ __launch_bounds__(TH) void part_scatter_kernel(const int32_t *in, int32_t *out, PartPlan pl, uint64_t n,
-                                                          const uint64_t *offsets)
+uint32_t grid_of(uint64_t work, uint32_t block, uint32_t cap)
 {
-    constexpr int kT = TH * kPerThread;
-    constexpr uint32_t kBinsPerThread = (kMaxBins + TH - 1) / TH;
-    __shared__ uint64_t cur[kMaxBins];      // next output slot of each bin
-    __shared__ uint32_t thist[kMaxBins];    // slots of each bin in this tile
-    __shared__ uint32_t tstart[kMaxBins];   // their exclusive scan
-    __shared__ int32_t tile[kT];
-    __shared__ uint64_t red[TH / 64];
-    if (blockIdx.x >= pl.seg_b0[pl.nseg]) return;
-    const BlockRange r = block_range(pl, blockIdx.x);
-    for (uint32_t bin = threadIdx.x; bin < pl.nbins; bin += TH) {
-        cur[bin] = offsets[r.count_base + (uint64_t)bin * r.nb + r.rank];
-        thist[bin] = 0;
+    uint64_t g = (work + block - 1) / block;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
+}
+
+// offsets[0, len] = exclusive scan of counts[0, len) with offsets[len] the
+// total; partials holds kScanParts entries.
+template <typename C>
+hipError_t scan_counts(const C *counts, uint64_t len, uint64_t *partials, uint64_t *offsets, hipStream_t stream)
+{
+    uint64_t chunk = kScanChunk;
+    while ((len + chunk - 1) / chunk > kScanParts) chunk *= 2;
+    const uint32_t parts = (uint32_t)std::max<uint64_t>(1, (len + chunk - 1) / chunk);
+    hipLaunchKernelGGL(scan_reduce_kernel<C>, dim3(parts), dim3(kThreads), 0, stream, counts, len, chunk, partials);
+    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kScanParts), 0, stream, partials, parts, offsets + len);
+    hipLaunchKernelGGL(scan_apply_kernel<C>, dim3(parts), dim3(kThreads), 0, stream, counts, len, chunk, partials,
+                       offsets);
+    return hipGetLastError();
+}
+
+hipError_t scan(const uint64_t *counts, uint64_t len, const DegreeWork &w, hipStream_t stream)
+{
+    return scan_counts(counts, len, w.partials, w.offsets, stream);
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// ---- degree pass --------------------------------------------------------------
+
+struct DegGeometry {
+    uint64_t first = 0, slots = 0;  // generated slots [first, first + slots) of G
+    uint32_t levels = 0;            // partition levels: 0, 1 or 2
+    uint32_t buckets = 0;           // level-1 bins
+    uint32_t shift = 0;             // level-1 bin of v: v >> shift
+    uint32_t blocks = 1;            // partition workgroups W
+    uint32_t leaves = 1;
+    uint64_t len1 = 0, len2 = 0;    // entries of the level-1 counts (buckets x W) and of the matrix (leaves x W)
+};
+
+DegGeometry deg_geometry(uint64_t n, uint32_t d, uint32_t num_cus)
+{
+    DegGeometry g;
+    g.first = (uint64_t)(d + 1) * d;
+    g.slots = n > (uint64_t)d + 1 ? (n - d - 1) * (uint64_t)d : 0;
+    // 4 partition workgroups per CU: the 512-thread scatters run in one round
+    const uint64_t b = std::min<uint64_t>((uint64_t)num_cus * kPartPerCU, kMaxBlocks);
+    g.blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, (g.slots + 65535) / 65536));
+    const uint64_t leaves = (n + kLeafNodes - 1) >> kLeafBits;
+    if (leaves <= 1) {
+        g.levels = 0;
+        g.leaves = 1;
+    } else if (leaves <= kMaxBuckets) {
+        g.levels = 1;
+        g.buckets = (uint32_t)leaves;
+        g.shift = kLeafBits;
+        g.leaves = g.buckets;
+        g.len1 = (uint64_t)g.buckets * g.blocks;
+    } else {
+        g.levels = 2;
+        g.buckets = (uint32_t)((n + (1ull << kBucketShift) - 1) >> kBucketShift);
+        g.shift = kBucketShift;
+        g.leaves = g.buckets * kSubBins;
+        g.len1 = (uint64_t)g.buckets * g.blocks;
+        g.len2 = (uint64_t)g.leaves * g.blocks;
     }
+    return g;
+}
+
+// Slots [lo, hi) of the input that partition workgroup w of W handles; the
+// count and the level-1 scatter share them.
+__device__ __forceinline__ void deg_range(uint64_t slots, uint32_t W, uint32_t w, uint64_t *lo, uint64_t *hi)
+{
+    const uint64_t per = (slots + W - 1) / W;
+    const uint64_t a = (uint64_t)w * per;
+    *lo = a < slots ? a : slots;
+    *hi = a + per < slots ? a + per : slots;
+}
+
+// Per workgroup w: c1[b * W + w] = its valid slots in level-1 bucket b and,
+// with MAT, cnt2[(b * 256 + s) * W + w] = those in leaf s of bucket b, for
+// the buckets [b_lo, b_lo + nb), nb <= 128. Cell 0 holds the smallest ids,
+// which in a preferential-attachment graph are named by far the most slots
+// (an eighth of them): it is counted per lane instead of contending for one
+// LDS word.
+template <bool MAT>
+__global__ __launch_bounds__(kCountThreads) void deg_count_kernel(const int32_t *in, uint64_t slots, uint64_t n,
+                                                                  uint32_t shift, uint32_t buckets, uint32_t b_lo,
+                                                                  uint32_t nb, uint32_t *c1, uint32_t *cnt2)
+{
+    constexpr uint32_t kCells = MAT ? kMatBuckets * kSubBins : kMaxBuckets;
+    constexpr int kT = kCountThreads * kPerThread;
+    __shared__ uint32_t cell[kCells];
+    const uint32_t W = gridDim.x, w = blockIdx.x;
+    uint64_t lo, hi;
+    deg_range(slots, W, w, &lo, &hi);
+    const uint32_t ncell = MAT ? nb * kSubBins : buckets;
+    for (uint32_t i = threadIdx.x; i < ncell; i += kCountThreads) cell[i] = 0;
     __syncthreads();
+    uint32_t hot = 0;
+    // kPerThread reads in flight per thread; the next tile's are issued
+    // before this tile's LDS updates
     int32_t v[kPerThread];
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
-        const uint64_t i = r.lo + (uint64_t)j * TH + threadIdx.x;
-        v[j] = i < r.hi ? in[i] : -1;
+        const uint64_t i = lo + (uint64_t)j * kCountThreads + threadIdx.x;
+        v[j] = i < hi ? in[i] : -1;
     }
-    for (uint64_t t0 = r.lo; t0 < r.hi; t0 += kT) {
-        uint32_t rank[kPerThread];
+    for (uint64_t t0 = lo; t0 < hi; t0 += kT) {
+        int32_t nv[kPerThread];
 #pragma unroll
         for (int j = 0; j < kPerThread; ++j) {
+            const uint64_t i = t0 + kT + (uint64_t)j * kCountThreads + threadIdx.x;
+            nv[j] = i < hi ? in[i] : -1;
+        }
+#pragma unroll
+        for (int j = 0; j < kPerThread; ++j) {
+            if (!valid(v[j], n)) continue;
+            const uint32_t u = (uint32_t)v[j];
+            uint32_t c = u >> shift;
+            if (MAT) {
+                const uint32_t b = c - b_lo;  // wraps below b_lo
+                if (b >= nb) continue;
+                c = b * kSubBins + ((u >> kLeafBits) & (kSubBins - 1));
+            }
+            if (c == 0) ++hot;
+            else atomicAdd(&cell[c], 1u);
+        }
+#pragma unroll
+        for (int j = 0; j < kPerThread; ++j) v[j] = nv[j];
+    }
+    hot = wave_sum(hot);
+    if ((threadIdx.x & 63u) == 0 && hot) atomicAdd(&cell[0], hot);
+    __syncthreads();
+    if (MAT) {
+        for (uint32_t i = threadIdx.x; i < ncell; i += kCountThreads)
+            cnt2[((uint64_t)b_lo * kSubBins + i) * W + w] = cell[i];
+        // bucket sums; thread b starts in its own bank
+        for (uint32_t b = threadIdx.x; b < nb; b += kCountThreads) {
+            uint32_t s = 0;
+            for (uint32_t k = 0; k < kSubBins; ++k) s += cell[b * kSubBins + ((k + b) & (kSubBins - 1))];
+            c1[(uint64_t)(b_lo + b) * W + w] = s;
+        }
+    } else {
+        for (uint32_t b = threadIdx.x; b < buckets; b += kCountThreads) c1[(uint64_t)b * W + w] = cell[b];
+    }
+}
+
+// Moves the valid keys of in[lo, hi) to their bins' runs of the output, one
+// 8192-slot tile at a time: the tile is grouped by bin ((v >> shift) & mask,
+// at most 256 bins) in LDS first, so each bin's run leaves the workgroup as
+// contiguous stores (at 128-256 bins a run averages 32-64 slots). cur[] holds
+// each bin's next output position and thist[] must be zero on entry; OUT16
+// stores a key's low 15 bits (the last level), otherwise the whole key. Bin
+// 0 is ranked with one LDS atomic per wavefront. Ends with a barrier.
+template <bool OUT16>
+__device__ __forceinline__ void scatter_run(const int32_t *in, uint64_t lo, uint64_t hi, uint64_t n, uint32_t shift,
+                                            uint32_t mask, uint32_t nbins, uint64_t *cur, uint32_t *thist,
+                                            uint32_t *tstart, int32_t *tile, uint64_t *red, int32_t *out32,
+                                            uint16_t *out16)
+{
+    int32_t v[kPartPer];
+#pragma unroll
+    for (int j = 0; j < kPartPer; ++j) {
+        const uint64_t i = lo + (uint64_t)j * kPartThreads + threadIdx.x;
+        v[j] = i < hi ? in[i] : -1;
+    }
+    const uint32_t b = threadIdx.x;  // this thread's bin in the per-tile scan
+    for (uint64_t t0 = lo; t0 < hi; t0 += kPartTile) {
+        uint32_t rank[kPartPer];
+#pragma unroll
+        for (int j = 0; j < kPartPer; ++j) {
             const bool ok = valid(v[j], n);
-            const uint32_t bin = ((uint32_t)v[j] >> pl.shift) & pl.mask;
-            // bin 0: one LDS atomic per wavefront, ranks by lane order
+            const uint32_t bin = ((uint32_t)v[j] >> shift) & mask;
             const uint64_t m0 = __ballot(ok && bin == 0);
             uint32_t base0 = 0;
             if (m0 != 0) {
@@ -357,84 +434,137 @@ __global__ __launch_bounds__(TH) void part_scatter_kernel(const int32_t *in, int
             rank[j] = !ok ? 0u : bin == 0 ? base0 + rank_below(m0) : atomicAdd(&thist[bin], 1u);
         }
         __syncthreads();
-        // bins [kBinsPerThread * tid, +kBinsPerThread) per thread
-        uint32_t c[kBinsPerThread], csum = 0;
-#pragma unroll
-        for (uint32_t q = 0; q < kBinsPerThread; ++q) {
-            const uint32_t b = kBinsPerThread * threadIdx.x + q;
-            c[q] = b < pl.nbins ? thist[b] : 0u;
-            csum += c[q];
-        }
+        const uint32_t c = b < nbins ? thist[b] : 0u;
         uint64_t tot = 0;
-        uint32_t ex = (uint32_t)block_exclusive_scan<TH>(csum, &tot, red);
-#pragma unroll
-        for (uint32_t q = 0; q < kBinsPerThread; ++q) {
-            const uint32_t b = kBinsPerThread * threadIdx.x + q;
-            if (b < pl.nbins) tstart[b] = ex;
-            ex += c[q];
-        }
+        const uint32_t ex = (uint32_t)block_exclusive_scan<kPartThreads>(c, &tot, red);
+        if (b < nbins) tstart[b] = ex;
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < kPerThread; ++j)
-            if (valid(v[j], n)) tile[tstart[((uint32_t)v[j] >> pl.shift) & pl.mask] + rank[j]] = v[j];
+        for (int j = 0; j < kPartPer; ++j)
+            if (valid(v[j], n)) tile[tstart[((uint32_t)v[j] >> shift) & mask] + rank[j]] = v[j];
         // the next tile's reads are in flight while this one is stored
 #pragma unroll
-        for (int j = 0; j < kPerThread; ++j) {
-            const uint64_t i = t0 + kT + (uint64_t)j * TH + threadIdx.x;
-            v[j] = i < r.hi ? in[i] : -1;
+        for (int j = 0; j < kPartPer; ++j) {
+            const uint64_t i = t0 + kPartTile + (uint64_t)j * kPartThreads + threadIdx.x;
+            v[j] = i < hi ? in[i] : -1;
         }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < (uint32_t)tot; i += TH) {
-            const int32_t w = tile[i];
-            const uint32_t bin = ((uint32_t)w >> pl.shift) & pl.mask;
-            out[cur[bin] + (i - tstart[bin])] = w;
+        for (uint32_t i = threadIdx.x; i < (uint32_t)tot; i += kPartThreads) {
+            const int32_t x = tile[i];
+            const uint32_t bin = ((uint32_t)x >> shift) & mask;
+            const uint64_t o = cur[bin] + (i - tstart[bin]);
+            if (OUT16) out16[o] = (uint16_t)((uint32_t)x & (kLeafNodes - 1));
+            else out32[o] = x;
         }
         __syncthreads();
-        for (uint32_t bin = threadIdx.x; bin < pl.nbins; bin += TH) {
-            cur[bin] += thist[bin];
-            thist[bin] = 0;
+        if (b < nbins) {
+            cur[b] += thist[b];
+            thist[b] = 0;
         }
         __syncthreads();
     }
 }
 
-// leaf_start = exclusive scan of leaf_size; piece_base = exclusive scan of
-// max(1, ceil(leaf_size / kPiece)); one workgroup.
-constexpr int kPlanThreads = 1024;
+// Level 1: workgroup w's slots (the count kernel's range) to the buckets,
+// its run of bucket b starting at offs1[b * W + w].
+template <bool OUT16>
+__global__ __launch_bounds__(kPartThreads) void deg_scatter1_kernel(const int32_t *in, uint64_t slots, uint64_t n,
+                                                                   uint32_t shift, uint32_t buckets,
+                                                                   const uint64_t *offs1, int32_t *out32,
+                                                                   uint16_t *out16)
+{
+    __shared__ uint64_t cur[kMaxBuckets];
+    __shared__ uint32_t thist[kMaxBuckets], tstart[kMaxBuckets];
+    __shared__ int32_t tile[kPartTile];
+    __shared__ uint64_t red[kPartThreads / 64];
+    const uint32_t W = gridDim.x, w = blockIdx.x;
+    uint64_t lo, hi;
+    deg_range(slots, W, w, &lo, &hi);
+    for (uint32_t b = threadIdx.x; b < buckets; b += kPartThreads) {
+        cur[b] = offs1[(uint64_t)b * W + w];
+        thist[b] = 0;
+    }
+    __syncthreads();
+    scatter_run<OUT16>(in, lo, hi, n, shift, kMaxBuckets - 1, buckets, cur, thist, tstart, tile, red, out32, out16);
+}
 
-__global__ __launch_bounds__(kPlanThreads) void leaf_plan_kernel(const uint64_t *leaf_size, uint32_t leaves,
-                                                                 uint64_t *leaf_start, uint64_t *piece_base)
+// Level 2: workgroup j moves the run that level-1 workgroup j left in each
+// bucket to the bucket's 256 leaves; the scanned matrix gives its run of
+// leaf s of bucket b at offs2[(b * 256 + s) * W + j].
+__global__ __launch_bounds__(kPartThreads) void deg_scatter2_kernel(const int32_t *part, uint64_t n, uint32_t buckets,
+                                                                   const uint64_t *offs1, const uint64_t *offs2,
+                                                                   uint16_t *out16)
+{
+    __shared__ uint64_t cur[kSubBins];
+    __shared__ uint32_t thist[kSubBins], tstart[kSubBins];
+    __shared__ int32_t tile[kPartTile];
+    __shared__ uint64_t red[kPartThreads / 64];
+    const uint32_t W = gridDim.x, j = blockIdx.x;
+    for (uint32_t b = 0; b < buckets; ++b) {
+        const uint64_t lo = offs1[(uint64_t)b * W + j], hi = offs1[(uint64_t)b * W + j + 1];
+        if (lo == hi) continue;
+        if (threadIdx.x < kSubBins) {
+            cur[threadIdx.x] = offs2[((uint64_t)b * kSubBins + threadIdx.x) * W + j];
+            thist[threadIdx.x] = 0;
+        }
+        __syncthreads();
+        scatter_run<true>(part, lo, hi, n, kLeafBits, kSubBins - 1, kSubBins, cur, thist, tstart, tile, red, nullptr,
+                          out16);
+    }
+}
+
+__global__ void deg_whole_kernel(uint64_t *offs, uint64_t slots)
+{
+    if (threadIdx.x == 0) {
+        offs[0] = 0;
+        offs[1] = slots;
+    }
+}
+
+__device__ __forceinline__ uint64_t pieces_of(uint64_t size) { return size > kPiece ? (size + kPiece - 1) / kPiece : 1; }
+
+// piece_base = exclusive scan over the leaves of their pieces, leaf l holding
+// the keys [offs[l * stride], offs[(l + 1) * stride]); one workgroup.
+__global__ __launch_bounds__(kPlanThreads) void deg_leaf_plan_kernel(const uint64_t *offs, uint32_t stride,
+                                                                    uint32_t leaves, uint64_t *piece_base)
 {
     __shared__ uint64_t red[kPlanThreads / 64];
     const uint32_t per = (leaves + kPlanThreads - 1) / kPlanThreads;
-    const uint32_t lo = threadIdx.x * per, hi = lo + per < leaves ? lo + per : leaves;
-    uint64_t s = 0, p = 0;
+    const uint32_t lo = std::min<uint32_t>(threadIdx.x * per, leaves), hi = std::min<uint32_t>(lo + per, leaves);
+    auto size = [&](uint32_t l) { return offs[(uint64_t)(l + 1) * stride] - offs[(uint64_t)l * stride]; };
+    uint64_t p = 0;
+    for (uint32_t l = lo; l < hi; ++l) p += pieces_of(size(l));
+    uint64_t tot = 0;
+    uint64_t run = block_exclusive_scan<kPlanThreads>(p, &tot, red);
     for (uint32_t l = lo; l < hi; ++l) {
-        const uint64_t z = leaf_size[l];
-        s += z;
-        p += z > kPiece ? (z + kPiece - 1) / kPiece : 1;
+        piece_base[l] = run;
+        run += pieces_of(size(l));
     }
-    uint64_t stot = 0, ptot = 0;
-    uint64_t srun = block_exclusive_scan<kPlanThreads>(s, &stot, red);
-    uint64_t prun = block_exclusive_scan<kPlanThreads>(p, &ptot, red);
-    for (uint32_t l = lo; l < hi; ++l) {
-        const uint64_t z = leaf_size[l];
-        leaf_start[l] = srun;
-        piece_base[l] = prun;
-        srun += z;
-        prun += z > kPiece ? (z + kPiece - 1) / kPiece : 1;
-    }
-    if (threadIdx.x == 0) {
-        leaf_start[leaves] = stot;
-        piece_base[leaves] = ptot;
+    if (threadIdx.x == 0) piece_base[leaves] = tot;
+}
+
+// deg = d over the leaves counted in several pieces, which add their counts
+// with atomics.
+__global__ void deg_hub_init_kernel(const uint64_t *piece_base, uint32_t leaves, uint64_t n, uint32_t d, uint32_t *deg)
+{
+    for (uint32_t l = blockIdx.x; l < leaves; l += gridDim.x) {
+        if (piece_base[l + 1] - piece_base[l] <= 1) continue;
+        const uint64_t base = (uint64_t)l << kLeafBits;
+        for (uint32_t i = threadIdx.x; i < kLeafNodes; i += blockDim.x)
+            if (base + i < n) deg[base + i] = d;
     }
 }
 
 // deg of every node of a leaf from LDS counters: a one-piece leaf stores
-// d + count, a split leaf adds its piece's counts to deg (set to d before).
-__global__ __launch_bounds__(kThreads) void leaf_count_kernel(const int32_t *in, uint64_t n, uint32_t d,
-                                                              uint32_t leaves, const uint64_t *leaf_start,
-                                                              const uint64_t *piece_base, uint32_t *deg)
+// d + count, a split leaf adds its piece's counts to deg. K16: the leaf keys
+// are 16-bit (read 8 per 16-byte load); otherwise the graph's own int32 slots
+// (a graph of at most 2^15 nodes, one leaf), which are checked. The leaf's
+// first node, a hub in leaf 0, is counted per lane.
+template <bool K16>
+__global__ __launch_bounds__(kLeafThreads) void deg_leaf_kernel(const void *keys, const uint64_t *offs,
+                                                               uint32_t stride, uint32_t leaves,
+                                                               const uint64_t *piece_base, uint64_t n, uint32_t d,
+                                                               uint32_t *deg)
 {
     __shared__ uint32_t cnt[kLeafNodes];
     const uint64_t pieces = piece_base[leaves];
@@ -446,24 +576,53 @@ __global__ __launch_bounds__(kThreads) void leaf_count_kernel(const int32_t *in,
             else hi = mid;
         }
         const uint64_t np = piece_base[lo + 1] - piece_base[lo];
-        const uint64_t first = leaf_start[lo] + (q - piece_base[lo]) * kPiece;
-        const uint64_t end = first + kPiece < leaf_start[lo + 1] ? first + kPiece : leaf_start[lo + 1];
-        for (uint32_t i = threadIdx.x; i < kLeafNodes; i += kThreads) cnt[i] = 0;
+        const uint64_t lend = offs[(uint64_t)(lo + 1) * stride];
+        const uint64_t first = offs[(uint64_t)lo * stride] + (q - piece_base[lo]) * kPiece;
+        const uint64_t end = first + kPiece < lend ? first + kPiece : lend;
+        for (uint32_t i = threadIdx.x; i < kLeafNodes; i += kLeafThreads) cnt[i] = 0;
         __syncthreads();
-        for (uint64_t t0 = first; t0 < end; t0 += kTile) {
-            int32_t v[kPerThread];
+        uint32_t hot = 0;
+        auto count = [&](uint32_t x) {
+            if (x == 0) ++hot;
+            else if (x < kLeafNodes) atomicAdd(&cnt[x], 1u);
+        };
+        if (K16) {
+            const uint16_t *k16 = static_cast<const uint16_t *>(keys);
+            const uint4 *k4 = static_cast<const uint4 *>(keys);  // 256-byte aligned buffer
+            const uint64_t a0 = std::min<uint64_t>(end, (first + 7) & ~7ull);  // head up to 8-key alignment
+            const uint64_t a1 = std::max<uint64_t>(a0, end & ~7ull);           // 16-byte body [a0, a1)
+            for (uint64_t i = first + threadIdx.x; i < a0; i += kLeafThreads) count(k16[i]);
+            const uint64_t g1 = a1 / 8;
+            for (uint64_t g = a0 / 8 + threadIdx.x; g < g1; g += 4 * kLeafThreads) {
+                uint4 x[4];
 #pragma unroll
-            for (int j = 0; j < kPerThread; ++j) {
-                const uint64_t i = t0 + (uint64_t)j * kThreads + threadIdx.x;
-                v[j] = i < end ? in[i] : -1;
+                for (int u = 0; u < 4; ++u) {
+                    const uint64_t gg = g + (uint64_t)u * kLeafThreads;
+                    x[u] = gg < g1 ? k4[gg] : make_uint4(~0u, ~0u, ~0u, ~0u);  // 0xffff: no key
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t wd[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        count(wd[h] & 0xffffu);
+                        count(wd[h] >> 16);
+                    }
+                }
             }
-#pragma unroll
-            for (int j = 0; j < kPerThread; ++j)
-                if (valid(v[j], n)) atomicAdd(&cnt[(uint32_t)v[j] & (kLeafNodes - 1)], 1u);
+            for (uint64_t i = a1 + threadIdx.x; i < end; i += kLeafThreads) count(k16[i]);
+        } else {
+            const int32_t *k32 = static_cast<const int32_t *>(keys);
+            for (uint64_t i = first + threadIdx.x; i < end; i += kLeafThreads) {
+                const int32_t v = k32[i];
+                if (valid(v, n)) count((uint32_t)v);
+            }
         }
+        hot = wave_sum(hot);
+        if ((threadIdx.x & 63u) == 0 && hot) atomicAdd(&cnt[0], hot);
         __syncthreads();
         const uint64_t base = (uint64_t)lo << kLeafBits;
-        for (uint32_t i = threadIdx.x; i < kLeafNodes; i += kThreads) {
+        for (uint32_t i = threadIdx.x; i < kLeafNodes; i += kLeafThreads) {
             const uint64_t v = base + i;
             if (v < n) {
                 if (np == 1) deg[v] = d + cnt[i];
@@ -473,37 +632,6 @@ __global__ __launch_bounds__(kThreads) void leaf_count_kernel(const int32_t *in,
         __syncthreads();
     }
 }
-
-uint32_t grid_of(uint64_t work, uint32_t block, uint32_t cap)
-{
-    uint64_t g = (work + block - 1) / block;
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
-}
-
-hipError_t scan(const uint64_t *counts, uint64_t len, const DegreeWork &w, hipStream_t stream)
-{
-    const uint32_t parts = (uint32_t)((len + kScanChunk - 1) / kScanChunk);
-    if (parts > kScanParts) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(scan_reduce_kernel, dim3(parts), dim3(kThreads), 0, stream, counts, len, w.partials);
-    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kScanParts), 0, stream, w.partials, parts, w.offsets + len);
-    hipLaunchKernelGGL(scan_apply_kernel, dim3(parts), dim3(kThreads), 0, stream, counts, len, w.partials,
-                       w.offsets);
-    return hipGetLastError();
-}
-
-// count, scan and scatter of one partition pass
-hipError_t partition(const int32_t *in, int32_t *out, const PartPlan &pl, uint32_t grid, uint64_t n,
-                     uint64_t count_len, uint64_t *leaf_size, const DegreeWork &w, hipStream_t stream)
-{
-    hipLaunchKernelGGL(part_count_kernel, dim3(grid), dim3(kThreads), 0, stream, in, pl, n, w.counts, leaf_size);
-    hipError_t e = scan(w.counts, count_len, w, stream);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(part_scatter_kernel<kScatterThreads>, dim3(grid), dim3(kScatterThreads), 0, stream, in, out, pl, n,
-                       (const uint64_t *)w.offsets);
-    return hipGetLastError();
-}
-
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // ---- in-edge CSR: a stable LSD radix sort of the slots by target ------------
 // Pass p orders the (target, source) pairs by bits [8p, 8p+8) of the target,
@@ -716,20 +844,19 @@ CsrGeometry csr_geometry(uint64_t n, uint32_t d, uint32_t num_cus)
 
 size_t degree_work_bytes(uint64_t n, uint32_t d, uint32_t num_cus)
 {
-    const Geometry g = geometry(n, d, num_cus);
+    const DegGeometry g = deg_geometry(n, d, num_cus);
     size_t b = 0;
-    if (g.levels >= 1) b += align256(g.slots * 4);
     if (g.levels == 2) b += align256(g.slots * 4);
-    b += 2 * align256(g.counts * 8);
-    b += align256(kScanParts * 8);
-    b += align256(2 * (kMaxBins + 1) * 4) + align256(2 * (kMaxBins + 1) * 8);
-    b += 3 * align256(((size_t)g.leaves + 1) * 8);
+    if (g.levels >= 1) b += align256(g.slots * 2);
+    b += align256(g.len1 * 4) + align256((g.len1 + 1) * 8);
+    b += align256(g.len2 * 4) + align256((g.len2 + 1) * 8);
+    b += align256(kScanParts * 8) + align256(2 * 8) + align256(((size_t)g.leaves + 1) * 8);
     return b;
 }
 
 void degree_work_layout(void *base, uint64_t n, uint32_t d, uint32_t num_cus, DegreeWork *w)
 {
-    const Geometry g = geometry(n, d, num_cus);
+    const DegGeometry g = deg_geometry(n, d, num_cus);
     char *p = (char *)base;
     auto take = [&](size_t bytes) {
         char *q = p;
@@ -737,15 +864,14 @@ void degree_work_layout(void *base, uint64_t n, uint32_t d, uint32_t num_cus, De
         return (void *)q;
     };
     *w = DegreeWork{};
-    if (g.levels >= 1) w->part_a = (int32_t *)take(g.slots * 4);
-    if (g.levels == 2) w->part_b = (int32_t *)take(g.slots * 4);
-    w->counts = (uint64_t *)take(g.counts * 8);
-    w->offsets = (uint64_t *)take(g.counts * 8);
+    if (g.levels == 2) w->part_a = (int32_t *)take(g.slots * 4);
+    if (g.levels >= 1) w->part_b = (uint16_t *)take(g.slots * 2);
+    w->c1 = (uint32_t *)take(g.len1 * 4);
+    w->offs1 = (uint64_t *)take((g.len1 + 1) * 8);
+    w->c2 = (uint32_t *)take(g.len2 * 4);
+    w->offs2 = (uint64_t *)take((g.len2 + 1) * 8);
     w->partials = (uint64_t *)take(kScanParts * 8);
-    w->seg_b0 = (uint32_t *)take(2 * (kMaxBins + 1) * 4);
-    w->seg_start = (uint64_t *)take(2 * (kMaxBins + 1) * 8);
-    w->leaf_size = (uint64_t *)take(((size_t)g.leaves + 1) * 8);
-    w->leaf_start = (uint64_t *)take(((size_t)g.leaves + 1) * 8);
+    w->whole = (uint64_t *)take(2 * 8);
     w->piece_base = (uint64_t *)take(((size_t)g.leaves + 1) * 8);
     w->num_cus = num_cus;
 }
@@ -753,44 +879,50 @@ void degree_work_layout(void *base, uint64_t n, uint32_t d, uint32_t num_cus, De
 hipError_t launch_degree_partitioned(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, const DegreeWork &w,
                                      hipStream_t stream)
 {
-    const Geometry g = geometry(n, d, w.num_cus);
-    hipLaunchKernelGGL(fill_deg_kernel, dim3(grid_of(n, kThreads, 8192)), dim3(kThreads), 0, stream, deg, n, d);
-    if (g.slots == 0) return hipGetLastError();
+    const DegGeometry g = deg_geometry(n, d, w.num_cus);
+    if (g.slots == 0) {
+        hipLaunchKernelGGL(fill_deg_kernel, dim3(grid_of(n, kThreads, 8192)), dim3(kThreads), 0, stream, deg, n, d);
+        return hipGetLastError();
+    }
     const int32_t *in = G + g.first;
-    const int32_t *leaf_in = in;
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_of(g.leaves, kThreads, 1024)), dim3(kThreads), 0, stream,
-                       w.leaf_size, (uint64_t)g.leaves, 0ull);
-    uint32_t *b0_1 = w.seg_b0, *b0_2 = w.seg_b0 + kMaxBins + 1;
-    uint64_t *st_1 = w.seg_start, *st_2 = w.seg_start + kMaxBins + 1;
+    const uint32_t W = g.blocks;
+    const uint64_t *offs = w.whole;
+    uint32_t stride = 1;
     hipError_t e = hipSuccess;
     if (g.levels == 0) {
-        hipLaunchKernelGGL(plan_whole_kernel, dim3(1), dim3(64), 0, stream, b0_1, st_1, 1u, g.slots, w.leaf_size);
+        hipLaunchKernelGGL(deg_whole_kernel, dim3(1), dim3(64), 0, stream, w.whole, g.slots);
     } else if (g.levels == 1) {
-        hipLaunchKernelGGL(plan_whole_kernel, dim3(1), dim3(64), 0, stream, b0_1, st_1, g.blocks1, g.slots,
-                           (uint64_t *)nullptr);
-        const PartPlan pl{b0_1, st_1, 1u, g.bins1, kLeafBits, 0xffu};
-        e = partition(in, w.part_a, pl, g.blocks1, n, (uint64_t)g.bins1 * g.blocks1, w.leaf_size, w, stream);
-        leaf_in = w.part_a;
+        hipLaunchKernelGGL(deg_count_kernel<false>, dim3(W), dim3(kCountThreads), 0, stream, in, g.slots, n, g.shift,
+                           g.buckets, 0u, 0u, w.c1, (uint32_t *)nullptr);
+        if ((e = scan_counts<uint32_t>(w.c1, g.len1, w.partials, w.offs1, stream)) != hipSuccess) return e;
+        hipLaunchKernelGGL(deg_scatter1_kernel<true>, dim3(W), dim3(kPartThreads), 0, stream, in, g.slots, n, g.shift,
+                           g.buckets, (const uint64_t *)w.offs1, (int32_t *)nullptr, w.part_b);
+        offs = w.offs1;
+        stride = W;
     } else {
-        hipLaunchKernelGGL(plan_whole_kernel, dim3(1), dim3(64), 0, stream, b0_1, st_1, g.blocks1, g.slots,
-                           (uint64_t *)nullptr);
-        const PartPlan p1{b0_1, st_1, 1u, g.bins1, kLevel1Shift, 0x1ffu};
-        e = partition(in, w.part_a, p1, g.blocks1, n, (uint64_t)g.bins1 * g.blocks1, nullptr, w, stream);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(plan_level2_kernel, dim3(1), dim3(kMaxBins), 0, stream, (const uint64_t *)w.offsets,
-                           g.blocks1, g.bins1, g.blocks2, b0_2, st_2);
-        const uint64_t len2 = (uint64_t)kLevel2Bins * g.blocks2;
-        hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_of(len2, kThreads, 1024)), dim3(kThreads), 0, stream,
-                           w.counts, len2, 0ull);
-        const PartPlan p2{b0_2, st_2, g.bins1, kLevel2Bins, kLeafBits, 0xffu};
-        e = partition(w.part_a, w.part_b, p2, g.blocks2, n, len2, w.leaf_size, w, stream);
-        leaf_in = w.part_b;
+        for (uint32_t b0 = 0; b0 < g.buckets; b0 += kMatBuckets)
+            hipLaunchKernelGGL(deg_count_kernel<true>, dim3(W), dim3(kCountThreads), 0, stream, in, g.slots, n,
+                               g.shift, g.buckets, b0, std::min<uint32_t>(kMatBuckets, g.buckets - b0), w.c1, w.c2);
+        if ((e = scan_counts<uint32_t>(w.c1, g.len1, w.partials, w.offs1, stream)) != hipSuccess) return e;
+        if ((e = scan_counts<uint32_t>(w.c2, g.len2, w.partials, w.offs2, stream)) != hipSuccess) return e;
+        hipLaunchKernelGGL(deg_scatter1_kernel<false>, dim3(W), dim3(kPartThreads), 0, stream, in, g.slots, n,
+                           g.shift, g.buckets, (const uint64_t *)w.offs1, w.part_a, (uint16_t *)nullptr);
+        hipLaunchKernelGGL(deg_scatter2_kernel, dim3(W), dim3(kPartThreads), 0, stream, (const int32_t *)w.part_a, n,
+                           g.buckets, (const uint64_t *)w.offs1, (const uint64_t *)w.offs2, w.part_b);
+        offs = w.offs2;
+        stride = W;
     }
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(leaf_plan_kernel, dim3(1), dim3(kPlanThreads), 0, stream, (const uint64_t *)w.leaf_size,
-                       g.leaves, w.leaf_start, w.piece_base);
-    hipLaunchKernelGGL(leaf_count_kernel, dim3(std::max<uint32_t>(1, 2 * w.num_cus)), dim3(kThreads), 0, stream,
-                       leaf_in, n, d, g.leaves, (const uint64_t *)w.leaf_start, (const uint64_t *)w.piece_base, deg);
+    hipLaunchKernelGGL(deg_leaf_plan_kernel, dim3(1), dim3(kPlanThreads), 0, stream, offs, stride, g.leaves,
+                       w.piece_base);
+    hipLaunchKernelGGL(deg_hub_init_kernel, dim3(std::min<uint32_t>(g.leaves, 1024)), dim3(kThreads), 0, stream,
+                       (const uint64_t *)w.piece_base, g.leaves, n, d, deg);
+    const uint32_t grid = std::max<uint32_t>(1, w.num_cus);
+    if (g.levels == 0)
+        hipLaunchKernelGGL(deg_leaf_kernel<false>, dim3(grid), dim3(kLeafThreads), 0, stream, (const void *)in, offs,
+                           stride, g.leaves, (const uint64_t *)w.piece_base, n, d, deg);
+    else
+        hipLaunchKernelGGL(deg_leaf_kernel<true>, dim3(grid), dim3(kLeafThreads), 0, stream, (const void *)w.part_b,
+                           offs, stride, g.leaves, (const uint64_t *)w.piece_base, n, d, deg);
     return hipGetLastError();
 }
 
@@ -822,16 +954,14 @@ hipError_t launch_in_csr(const int32_t *G, uint64_t n, uint32_t d, int64_t *in_p
     const uint64_t len = (uint64_t)kRadixBins * g.blocks;
     const double inv_d = 1.0 / d;
     hipError_t e = hipSuccess;
-    hipLaunchKernelGGL(plan_whole_kernel, dim3(1), dim3(64), 0, stream, seg_b0, seg_start, g.blocks, g.slots,
-                       (uint64_t *)nullptr);
+    hipLaunchKernelGGL(plan_whole_kernel, dim3(1), dim3(64), 0, stream, seg_b0, seg_start, g.blocks, g.slots);
     const int32_t *kin = G, *vin = nullptr;
     for (uint32_t pass = 0; pass < g.passes; ++pass) {
         const PartPlan pl{seg_b0, seg_start, 1u, kRadixBins, pass * kRadixBits, kRadixBins - 1};
         const bool last = pass + 1 == g.passes;
         int32_t *kout = (pass & 1) ? kb : ka;
         int32_t *vout = last ? in_src : ((pass & 1) ? vb : va);
-        hipLaunchKernelGGL(part_count_kernel, dim3(g.blocks), dim3(kThreads), 0, stream, kin, pl, n, w.counts,
-                           (uint64_t *)nullptr);
+        hipLaunchKernelGGL(part_count_kernel, dim3(g.blocks), dim3(kThreads), 0, stream, kin, pl, n, w.counts);
         if ((e = scan(w.counts, len, w, stream)) != hipSuccess) return e;
         hipLaunchKernelGGL(lsd_scatter_kernel<kLsdThreads>, dim3(g.blocks), dim3(kLsdThreads), 0, stream, kin, vin, kout,
                            vout, pl, n,

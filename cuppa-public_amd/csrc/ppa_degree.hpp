@@ -7,18 +7,20 @@
 namespace ppa {
 
 // Device buffers of one partitioned degree pass, carved from one allocation
-// by degree_work_layout.
+// by degree_work_layout (the in-edge CSR borrows counts, offsets and
+// partials for its radix passes).
 struct DegreeWork {
-    int32_t *part_a = nullptr;      // slots after the first partition
-    int32_t *part_b = nullptr;      // slots after the second partition (two-level graphs)
-    uint64_t *counts = nullptr;     // per (segment, bin, workgroup) counts
-    uint64_t *offsets = nullptr;    // their exclusive scan, one extra entry for the total
+    int32_t *part_a = nullptr;      // slots after the level-1 partition (graphs of more than 2^23 nodes)
+    uint16_t *part_b = nullptr;     // slots in leaf order, low 15 bits of each target
+    uint32_t *c1 = nullptr;         // level-1 counts per (bucket, workgroup)
+    uint64_t *offs1 = nullptr;      // their exclusive scan, one extra entry for the total
+    uint32_t *c2 = nullptr;         // level-2 counts per (bucket, leaf, workgroup)
+    uint64_t *offs2 = nullptr;      // their exclusive scan, one extra entry for the total
     uint64_t *partials = nullptr;   // scan workgroup sums
-    uint32_t *seg_b0 = nullptr;     // 2 levels x (segments + 1): first workgroup per segment
-    uint64_t *seg_start = nullptr;  // 2 levels x (segments + 1): first slot per segment
-    uint64_t *leaf_size = nullptr;  // slots per leaf of 2^14 nodes
-    uint64_t *leaf_start = nullptr; // their exclusive scan
+    uint64_t *whole = nullptr;      // {0, slots}: the one leaf of a graph of at most 2^15 nodes
     uint64_t *piece_base = nullptr; // exclusive scan of pieces per leaf
+    uint64_t *counts = nullptr;     // in-edge CSR: per (digit, workgroup) counts
+    uint64_t *offsets = nullptr;    // in-edge CSR: their exclusive scan
     uint32_t num_cus = 0;           // compute units the launch shapes are sized for
 };
 
