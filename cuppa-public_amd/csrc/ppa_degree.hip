@@ -162,19 +162,25 @@ __global__ __launch_bounds__(kThreads) void part_count_kernel(const int32_t *in,
     // kPerThread reads in flight per thread; the next tile's are issued
     // before this tile's LDS updates
     uint32_t hot = 0;  // this lane's slots in bin 0
-    int32_t v[kPerThread];
+    // a tile inside the range loads from a uniform base with 32-bit offsets
+    auto load_tile = [&](uint64_t t, int32_t *x) {
+        if (t + kTile <= r.hi) {
+            const int32_t *b = in + t;
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
-        const uint64_t i = r.lo + (uint64_t)j * kThreads + threadIdx.x;
-        v[j] = i < r.hi ? in[i] : -1;
-    }
+            for (int j = 0; j < kPerThread; ++j) x[j] = b[j * kThreads + threadIdx.x];
+        } else {
+#pragma unroll
+            for (int j = 0; j < kPerThread; ++j) {
+                const uint64_t i = t + (uint64_t)j * kThreads + threadIdx.x;
+                x[j] = i < r.hi ? in[i] : -1;
+            }
+        }
+    };
+    int32_t v[kPerThread];
+    load_tile(r.lo, v);
     for (uint64_t t0 = r.lo; t0 < r.hi; t0 += kTile) {
         int32_t nv[kPerThread];
-#pragma unroll
-        for (int j = 0; j < kPerThread; ++j) {
-            const uint64_t i = t0 + kTile + (uint64_t)j * kThreads + threadIdx.x;
-            nv[j] = i < r.hi ? in[i] : -1;
-        }
+        load_tile(t0 + kTile, nv);
 #pragma unroll
         for (int j = 0; j < kPerThread; ++j) {
             if (valid(v[j], n)) {
@@ -351,20 +357,26 @@ __global__ __launch_bounds__(kCountThreads) void deg_count_kernel(const int32_t 
     __syncthreads();
     uint32_t hot = 0;
     // kPerThread reads in flight per thread; the next tile's are issued
-    // before this tile's LDS updates
-    int32_t v[kPerThread];
+    // before this tile's LDS updates. A tile inside the range loads from a
+    // uniform base with 32-bit offsets.
+    auto load_tile = [&](uint64_t t, int32_t *x) {
+        if (t + kT <= hi) {
+            const int32_t *b = in + t;
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
-        const uint64_t i = lo + (uint64_t)j * kCountThreads + threadIdx.x;
-        v[j] = i < hi ? in[i] : -1;
-    }
+            for (int j = 0; j < kPerThread; ++j) x[j] = b[j * kCountThreads + threadIdx.x];
+        } else {
+#pragma unroll
+            for (int j = 0; j < kPerThread; ++j) {
+                const uint64_t i = t + (uint64_t)j * kCountThreads + threadIdx.x;
+                x[j] = i < hi ? in[i] : -1;
+            }
+        }
+    };
+    int32_t v[kPerThread];
+    load_tile(lo, v);
     for (uint64_t t0 = lo; t0 < hi; t0 += kT) {
         int32_t nv[kPerThread];
-#pragma unroll
-        for (int j = 0; j < kPerThread; ++j) {
-            const uint64_t i = t0 + kT + (uint64_t)j * kCountThreads + threadIdx.x;
-            nv[j] = i < hi ? in[i] : -1;
-        }
+        load_tile(t0 + kT, nv);
 #pragma unroll
         for (int j = 0; j < kPerThread; ++j) {
             if (!valid(v[j], n)) continue;
@@ -411,12 +423,22 @@ __device__ __forceinline__ void scatter_run(const int32_t *in, uint64_t lo, uint
                                             uint32_t *tstart, int32_t *tile, uint64_t *red, int32_t *out32,
                                             uint16_t *out16)
 {
-    int32_t v[kPartPer];
+    // a tile inside the range loads from a uniform base with 32-bit offsets
+    auto load_tile = [&](uint64_t t, int32_t *x) {
+        if (t + kPartTile <= hi) {
+            const int32_t *p = in + t;
 #pragma unroll
-    for (int j = 0; j < kPartPer; ++j) {
-        const uint64_t i = lo + (uint64_t)j * kPartThreads + threadIdx.x;
-        v[j] = i < hi ? in[i] : -1;
-    }
+            for (int j = 0; j < kPartPer; ++j) x[j] = p[j * kPartThreads + threadIdx.x];
+        } else {
+#pragma unroll
+            for (int j = 0; j < kPartPer; ++j) {
+                const uint64_t i = t + (uint64_t)j * kPartThreads + threadIdx.x;
+                x[j] = i < hi ? in[i] : -1;
+            }
+        }
+    };
+    int32_t v[kPartPer];
+    load_tile(lo, v);
     const uint32_t b = threadIdx.x;  // this thread's bin in the per-tile scan
     for (uint64_t t0 = lo; t0 < hi; t0 += kPartTile) {
         uint32_t rank[kPartPer];
@@ -443,11 +465,7 @@ __device__ __forceinline__ void scatter_run(const int32_t *in, uint64_t lo, uint
         for (int j = 0; j < kPartPer; ++j)
             if (valid(v[j], n)) tile[tstart[((uint32_t)v[j] >> shift) & mask] + rank[j]] = v[j];
         // the next tile's reads are in flight while this one is stored
-#pragma unroll
-        for (int j = 0; j < kPartPer; ++j) {
-            const uint64_t i = t0 + kPartTile + (uint64_t)j * kPartThreads + threadIdx.x;
-            v[j] = i < hi ? in[i] : -1;
-        }
+        load_tile(t0 + kPartTile, v);
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < (uint32_t)tot; i += kPartThreads) {
             const int32_t x = tile[i];
@@ -674,13 +692,17 @@ __global__ void plan_total_kernel(uint32_t *seg_b0, uint64_t *seg_start, uint32_
     }
 }
 
-// One stable pass. vin == nullptr: the sources are the slot indices' rows,
-// (first + i) / d (first pass, reading the targets array itself). kout may be
-// null (no later pass needs the keys).
-template <uint32_t TH>
-__global__ __launch_bounds__(TH) void lsd_scatter_kernel(const int32_t *kin, const int32_t *vin, int32_t *kout,
+// One stable pass. FIRST: the keys are the targets array itself and the
+// sources the slots' rows i / d (vin unused); otherwise both are read. The
+// keys are written for the next pass (the last pass's, for in_ptr). A wavefront whose
+// 1024 slots lie inside the range loads them from a wavefront-uniform base
+// with 32-bit offsets and no bound checks; the first pass steps each lane's
+// row i / d by 64 slots at a time (qstep = 64 / d, rstep = 64 % d).
+template <uint32_t TH, bool FIRST>
+__global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void lsd_scatter_kernel(const int32_t *kin, const int32_t *vin, int32_t *kout,
                                                          int32_t *vout, PartPlan pl, uint64_t n, uint32_t d,
-                                                         double inv_d, uint64_t first, const uint64_t *offsets)
+                                                         double inv_d, uint32_t qstep, uint32_t rstep,
+                                                         const uint64_t *offsets)
 {
     constexpr uint32_t kWaves = TH / 64;
     constexpr uint32_t kT = kWaves * kWaveShare;
@@ -692,7 +714,8 @@ __global__ __launch_bounds__(TH) void lsd_scatter_kernel(const int32_t *kin, con
     __shared__ uint64_t red[kWaves];
     if (blockIdx.x >= pl.seg_b0[pl.nseg]) return;
     const BlockRange r = block_range(pl, blockIdx.x);
-    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const bool digit_thread = threadIdx.x < kRadixBins;
     if (digit_thread) cur[threadIdx.x] = offsets[r.count_base + (uint64_t)threadIdx.x * r.nb + r.rank];
     __syncthreads();
@@ -700,11 +723,36 @@ __global__ __launch_bounds__(TH) void lsd_scatter_kernel(const int32_t *kin, con
         for (uint32_t b = lane; b < kRadixBins; b += 64) wrun[wave][b] = 0;
         int32_t k[kSteps], v[kSteps];
         uint32_t rk[kSteps];
+        const uint64_t w0 = t0 + (uint64_t)wave * kWaveShare;  // this wavefront's first slot
+        if (w0 + kWaveShare <= r.hi) {
+            const int32_t *kb = kin + w0;
+            const int32_t *vb = vin + w0;
 #pragma unroll
-        for (uint32_t j = 0; j < kSteps; ++j) {
-            const uint64_t i = t0 + wave * kWaveShare + j * 64 + lane;
-            k[j] = i < r.hi ? kin[i] : -1;
-            v[j] = vin ? (i < r.hi ? vin[i] : 0) : (int32_t)div_small(first + i, d, inv_d);
+            for (uint32_t j = 0; j < kSteps; ++j) {
+                k[j] = kb[j * 64 + lane];
+                if (!FIRST) v[j] = vb[j * 64 + lane];
+            }
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < kSteps; ++j) {
+                const uint64_t i = w0 + j * 64 + lane;
+                k[j] = i < r.hi ? kin[i] : -1;
+                if (!FIRST) v[j] = i < r.hi ? vin[i] : 0;
+            }
+        }
+        if (FIRST) {
+            uint32_t q = (uint32_t)div_small(w0 + lane, d, inv_d);
+            uint32_t rem = (uint32_t)(w0 + lane - (uint64_t)q * d);
+#pragma unroll
+            for (uint32_t j = 0; j < kSteps; ++j) {
+                v[j] = (int32_t)q;
+                q += qstep;
+                rem += rstep;
+                if (rem >= d) {
+                    rem -= d;
+                    ++q;
+                }
+            }
         }
 #pragma unroll
         for (uint32_t j = 0; j < kSteps; ++j) {
@@ -747,7 +795,7 @@ __global__ __launch_bounds__(TH) void lsd_scatter_kernel(const int32_t *kin, con
             const int32_t kk = tk[i];
             const uint32_t dig = ((uint32_t)kk >> pl.shift) & (kRadixBins - 1);
             const uint64_t o = cur[dig] + (i - tstart[dig]);
-            if (kout) kout[o] = kk;
+            kout[o] = kk;
             vout[o] = tv[i];
         }
         __syncthreads();
@@ -963,9 +1011,15 @@ hipError_t launch_in_csr(const int32_t *G, uint64_t n, uint32_t d, int64_t *in_p
         int32_t *vout = last ? in_src : ((pass & 1) ? vb : va);
         hipLaunchKernelGGL(part_count_kernel, dim3(g.blocks), dim3(kThreads), 0, stream, kin, pl, n, w.counts);
         if ((e = scan(w.counts, len, w, stream)) != hipSuccess) return e;
-        hipLaunchKernelGGL(lsd_scatter_kernel<kLsdThreads>, dim3(g.blocks), dim3(kLsdThreads), 0, stream, kin, vin, kout,
-                           vout, pl, n,
-                           d, inv_d, 0ull, (const uint64_t *)w.offsets);
+        const uint32_t qstep = 64 / d, rstep = 64 % d;
+        if (pass == 0)
+            hipLaunchKernelGGL((lsd_scatter_kernel<kLsdThreads, true>), dim3(g.blocks), dim3(kLsdThreads), 0,
+                               stream, kin, vin, kout, vout, pl, n, d, inv_d, qstep, rstep,
+                               (const uint64_t *)w.offsets);
+        else
+            hipLaunchKernelGGL((lsd_scatter_kernel<kLsdThreads, false>), dim3(g.blocks), dim3(kLsdThreads), 0,
+                               stream, kin, vin, kout, vout, pl, n, d, inv_d, qstep, rstep,
+                               (const uint64_t *)w.offsets);
         if (pass == 0)  // later passes cover the valid pairs only
             hipLaunchKernelGGL(plan_total_kernel, dim3(1), dim3(64), 0, stream, seg_b0, seg_start, g.blocks,
                                (const uint64_t *)(w.offsets + len));
