@@ -573,8 +573,9 @@ def main():
                          "small and mid-size graphs; wide: single graphs of out-degree 32 .. 8192")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-runs", type=int, default=3, help="timed passes of each side (median, >= 3)")
-    ap.add_argument("--batch-autotune", action="store_true",
-                    help="run the autotuner on the ensemble first (about 25 extra launches)")
+    ap.add_argument("--no-batch-autotune", dest="batch_autotune", action="store_false",
+                    help="time the batch with the default launch shape instead of running the autotuner "
+                         "on the ensemble first (about 25 extra launches, ~20 s)")
     ap.add_argument("--serve-count", type=int, default=10_000,
                     help="requests of each replay, plain and served alike")
     ap.add_argument("--serve-budget-gb", type=int, default=200)
