@@ -12,7 +12,10 @@ Config 2 (n = 2^28, d = 8, p = 0.5, seed = 7):
     recount, sum(hist) = n and sum(deg * hist) = d(d+1) + 2(n-d-1)d.
 Config 3 (n = 2^30, d = 8, p = 0.2, seed = 11): the first 2^22 rows bit for
 bit against the sequential oracle, and every row through the
-row-consistency verifier; the stats' node count.
+row-consistency verifier; the stats' node count. Its degree pass and in-edge
+CSR (2^33 slots: positions past 2^32) against torch recounts: every degree,
+every in-list's length, every list ascending, the sources' multiset through
+two checksums, and the exact lists of hub and tail nodes.
 """
 import os
 
@@ -131,6 +134,70 @@ def test_n2p30_every_row_consistent(ctx, big_graph, oracle_mod):
     print("[scale] verifying every row of n=2^30", flush=True)
     bad = oracle_mod.verify_rows(n, d, T, seed, host, threads=THREADS)
     assert bad is None, f"first inconsistent row {bad}"
+
+
+@pytest.mark.timeout(900)
+def test_n2p30_degree_and_in_csr(ctx):
+    import torch
+    n, d, p, seed = 1 << 30, 8, 0.2, 11
+    ctx.trim()
+    torch.cuda.empty_cache()
+    print(f"\n[scale] generating n={n} d={d} p={p} seed={seed}", flush=True)
+    g = ctx.generate(n, d, p, seed, row_ptr=False)
+    tg = g.targets
+    step = 1 << 28
+    # degrees: deg(v) = d + slots of nodes t >= d+1 naming v
+    print("[scale] degree pass against torch.bincount", flush=True)
+    hist, mx, deg = ctx.degree_histogram(g, want_degree=True)
+    cnt = torch.zeros(n, dtype=torch.int64, device=tg.device)
+    first = (d + 1) * d
+    for lo in range(first, tg.numel(), step):
+        cnt += torch.bincount(tg[lo:min(lo + step, tg.numel())].to(torch.int64), minlength=n)
+    assert bool((deg.to(torch.int64) == cnt + d).all())
+    assert mx == int(cnt.max()) + d
+    assert torch.equal(hist, torch.bincount(cnt + d))
+    del deg, hist
+    ctx.trim()
+    cnt += torch.bincount(tg[:first].to(torch.int64), minlength=n)  # in-edges of all slots
+    # exact lists of a few nodes, from the targets themselves (a row names v at most once)
+    probe = [0, 1, d, d + 1, 1000, 1 << 20, n // 2, n - 1]
+    # (torch.nonzero of more than 2^32 elements overflows: chunks of 2^28)
+    want_lists = {}
+    for v in probe:
+        parts = [torch.nonzero(tg[lo:lo + step] == v).flatten() + lo for lo in range(0, tg.numel(), step)]
+        want_lists[v] = (torch.cat(parts) // d).to(torch.int32).cpu()
+    print("[scale] in-edge CSR", flush=True)
+    in_ptr, in_src = ctx.in_csr(g)
+    ctx.trim()
+    del g, tg
+    torch.cuda.empty_cache()
+    total = n * d
+    assert int(in_ptr[0]) == 0 and int(in_ptr[n]) == total
+    assert torch.equal(in_ptr[1:] - in_ptr[:-1], cnt)
+    del cnt
+    for v in probe:
+        assert torch.equal(in_src[int(in_ptr[v]):int(in_ptr[v + 1])].cpu(), want_lists[v]), f"in-list of {v}"
+    # every list strictly ascending: a non-increasing step only where a list starts
+    for lo in range(0, total - 1, step):
+        hi = min(lo + step, total - 1)
+        x = in_src[lo:hi + 1].to(torch.int64)
+        bad = (x[1:] - x[:-1]) <= 0  # position lo + 1 + i
+        a = int(torch.searchsorted(in_ptr, torch.tensor([lo + 1], device=in_ptr.device)))
+        b = int(torch.searchsorted(in_ptr, torch.tensor([hi + 1], device=in_ptr.device)))
+        starts = in_ptr[a:b] - (lo + 1)
+        bad[starts] = False
+        assert not bool(bad.any()), f"a list is not ascending in [{lo}, {hi}]"
+    # each node is the source of exactly d slots: two checksums of the sources
+    s1 = s2 = 0
+    for lo in range(0, total, step):
+        x = in_src[lo:lo + step].to(torch.int64)
+        s1 += int(x.sum())
+        s2 += int(((x & 4095) ** 2).sum())
+    t = torch.arange(n, dtype=torch.int64, device=in_src.device)
+    assert s1 == d * (n * (n - 1) // 2)
+    assert s2 == d * int(((t & 4095) ** 2).sum())
+    del in_ptr, in_src, t
+    torch.cuda.empty_cache()
 
 
 def test_d16_n2p24_every_row_consistent(ctx, oracle_mod):
