@@ -50,6 +50,7 @@ constexpr uint32_t kSubBins = 1u << kSubBits;
 constexpr uint32_t kBucketShift = kLeafBits + kSubBits;  // two-level bucket: v >> 23
 constexpr uint32_t kMaxBuckets = 256;         // n <= 2^31
 constexpr uint32_t kMatBuckets = 128;         // buckets one matrix count covers (128 KB of LDS)
+constexpr uint32_t kHubNodes = 8192;          // nodes the count pass counts itself (32 KB of LDS)
 constexpr int kCountThreads = 1024;           // count workgroups: one per CU by LDS, 16 waves
 constexpr int kPartThreads = 512;             // scatter workgroups: 8192-slot tiles
 constexpr int kPartPer = 16;                  // slots per scatter thread per tile
@@ -338,25 +339,30 @@ __device__ __forceinline__ void deg_range(uint64_t slots, uint32_t W, uint32_t w
 
 // Per workgroup w: c1[b * W + w] = its valid slots in level-1 bucket b and,
 // with MAT, cnt2[(b * 256 + s) * W + w] = those in leaf s of bucket b, for
-// the buckets [b_lo, b_lo + nb), nb <= 128. Cell 0 holds the smallest ids,
-// which in a preferential-attachment graph are named by far the most slots
-// (an eighth of them): it is counted per lane instead of contending for one
-// LDS word.
+// the buckets [b_lo, b_lo + nb), nb <= 128. The hub nodes v < hub_n (the
+// smallest ids, which a preferential-attachment graph's slots name most: a
+// tenth of them below 8192 at n = 2^30) are counted here, exactly, in LDS and
+// added to hub[v], and are left out of the buckets, so the scatters and the
+// leaf counts never see them. Node 0 and cell 0 are counted per lane instead
+// of contending for one LDS word.
 template <bool MAT>
 __global__ __launch_bounds__(kCountThreads) void deg_count_kernel(const int32_t *in, uint64_t slots, uint64_t n,
                                                                   uint32_t shift, uint32_t buckets, uint32_t b_lo,
-                                                                  uint32_t nb, uint32_t *c1, uint32_t *cnt2)
+                                                                  uint32_t nb, uint32_t hub_n, uint32_t *c1,
+                                                                  uint32_t *cnt2, uint32_t *hub)
 {
     constexpr uint32_t kCells = MAT ? kMatBuckets * kSubBins : kMaxBuckets;
     constexpr int kT = kCountThreads * kPerThread;
     __shared__ uint32_t cell[kCells];
+    __shared__ uint32_t hubc[kHubNodes];
     const uint32_t W = gridDim.x, w = blockIdx.x;
     uint64_t lo, hi;
     deg_range(slots, W, w, &lo, &hi);
     const uint32_t ncell = MAT ? nb * kSubBins : buckets;
     for (uint32_t i = threadIdx.x; i < ncell; i += kCountThreads) cell[i] = 0;
+    for (uint32_t i = threadIdx.x; i < hub_n; i += kCountThreads) hubc[i] = 0;
     __syncthreads();
-    uint32_t hot = 0;
+    uint32_t hot = 0, hot_hub = 0;
     // kPerThread reads in flight per thread; the next tile's are issued
     // before this tile's LDS updates. A tile inside the range loads from a
     // uniform base with 32-bit offsets.
@@ -382,6 +388,11 @@ __global__ __launch_bounds__(kCountThreads) void deg_count_kernel(const int32_t 
         for (int j = 0; j < kPerThread; ++j) {
             if (!valid(v[j], n)) continue;
             const uint32_t u = (uint32_t)v[j];
+            if (u < hub_n) {
+                if (u == 0) ++hot_hub;
+                else atomicAdd(&hubc[u], 1u);
+                continue;
+            }
             uint32_t c = u >> shift;
             if (MAT) {
                 const uint32_t b = c - b_lo;  // wraps below b_lo
@@ -396,7 +407,11 @@ __global__ __launch_bounds__(kCountThreads) void deg_count_kernel(const int32_t 
     }
     hot = wave_sum(hot);
     if ((threadIdx.x & 63u) == 0 && hot) atomicAdd(&cell[0], hot);
+    hot_hub = wave_sum(hot_hub);
+    if ((threadIdx.x & 63u) == 0 && hot_hub) atomicAdd(&hubc[0], hot_hub);
     __syncthreads();
+    for (uint32_t i = threadIdx.x; i < hub_n; i += kCountThreads)
+        if (hubc[i]) atomicAdd(&hub[i], hubc[i]);
     if (MAT) {
         for (uint32_t i = threadIdx.x; i < ncell; i += kCountThreads)
             cnt2[((uint64_t)b_lo * kSubBins + i) * W + w] = cell[i];
@@ -419,11 +434,13 @@ __global__ __launch_bounds__(kCountThreads) void deg_count_kernel(const int32_t 
 // stores a key's low 15 bits (the last level), otherwise the whole key. Bin
 // 0 is ranked with one LDS atomic per wavefront. Ends with a barrier.
 template <bool OUT16>
-__device__ __forceinline__ void scatter_run(const int32_t *in, uint64_t lo, uint64_t hi, uint64_t n, uint32_t shift,
-                                            uint32_t mask, uint32_t nbins, uint64_t *cur, uint32_t *thist,
-                                            uint32_t *tstart, int32_t *tile, uint64_t *red, int32_t *out32,
-                                            uint16_t *out16)
+__device__ __forceinline__ void scatter_run(const int32_t *in, uint64_t lo, uint64_t hi, uint64_t n, uint32_t key_lo,
+                                            uint32_t shift, uint32_t mask, uint32_t nbins, uint64_t *cur,
+                                            uint32_t *thist, uint32_t *tstart, int32_t *tile, uint64_t *red,
+                                            int32_t *out32, uint16_t *out16)
 {
+    // keys in [key_lo, n) move; the rest (out of range, or hub nodes the count pass took) stay
+    auto keep = [&](int32_t x) { return valid(x, n) && (uint32_t)x >= key_lo; };
     // a tile inside the range loads from a uniform base with 32-bit offsets
     auto load_tile = [&](uint64_t t, int32_t *x) {
         if (t + kPartTile <= hi) {
@@ -445,7 +462,7 @@ __device__ __forceinline__ void scatter_run(const int32_t *in, uint64_t lo, uint
         uint32_t rank[kPartPer];
 #pragma unroll
         for (int j = 0; j < kPartPer; ++j) {
-            const bool ok = valid(v[j], n);
+            const bool ok = keep(v[j]);
             const uint32_t bin = ((uint32_t)v[j] >> shift) & mask;
             const uint64_t m0 = __ballot(ok && bin == 0);
             uint32_t base0 = 0;
@@ -464,7 +481,7 @@ __device__ __forceinline__ void scatter_run(const int32_t *in, uint64_t lo, uint
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kPartPer; ++j)
-            if (valid(v[j], n)) tile[tstart[((uint32_t)v[j] >> shift) & mask] + rank[j]] = v[j];
+            if (keep(v[j])) tile[tstart[((uint32_t)v[j] >> shift) & mask] + rank[j]] = v[j];
         // the next tile's reads are in flight while this one is stored
         load_tile(t0 + kPartTile, v);
         __syncthreads();
@@ -488,7 +505,7 @@ __device__ __forceinline__ void scatter_run(const int32_t *in, uint64_t lo, uint
 // its run of bucket b starting at offs1[b * W + w].
 template <bool OUT16>
 __global__ __launch_bounds__(kPartThreads) void deg_scatter1_kernel(const int32_t *in, uint64_t slots, uint64_t n,
-                                                                   uint32_t shift, uint32_t buckets,
+                                                                   uint32_t hub_n, uint32_t shift, uint32_t buckets,
                                                                    const uint64_t *offs1, int32_t *out32,
                                                                    uint16_t *out16)
 {
@@ -504,7 +521,8 @@ __global__ __launch_bounds__(kPartThreads) void deg_scatter1_kernel(const int32_
         thist[b] = 0;
     }
     __syncthreads();
-    scatter_run<OUT16>(in, lo, hi, n, shift, kMaxBuckets - 1, buckets, cur, thist, tstart, tile, red, out32, out16);
+    scatter_run<OUT16>(in, lo, hi, n, hub_n, shift, kMaxBuckets - 1, buckets, cur, thist, tstart, tile, red, out32,
+                       out16);
 }
 
 // Level 2: workgroup j moves the run that level-1 workgroup j left in each
@@ -527,8 +545,8 @@ __global__ __launch_bounds__(kPartThreads) void deg_scatter2_kernel(const int32_
             thist[threadIdx.x] = 0;
         }
         __syncthreads();
-        scatter_run<true>(part, lo, hi, n, kLeafBits, kSubBins - 1, kSubBins, cur, thist, tstart, tile, red, nullptr,
-                          out16);
+        scatter_run<true>(part, lo, hi, n, 0u, kLeafBits, kSubBins - 1, kSubBins, cur, thist, tstart, tile, red,
+                          nullptr, out16);
     }
 }
 
@@ -572,6 +590,19 @@ __global__ void deg_hub_init_kernel(const uint64_t *piece_base, uint32_t leaves,
         for (uint32_t i = threadIdx.x; i < kLeafNodes; i += blockDim.x)
             if (base + i < n) deg[base + i] = d;
     }
+}
+
+// The hub nodes' counts from the count pass, added after the leaf counts
+// (which never saw their slots).
+__global__ void deg_hub_add_kernel(const uint32_t *hub, uint32_t hub_n, uint32_t *deg)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < hub_n; v += gridDim.x * blockDim.x)
+        deg[v] += hub[v];
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, uint32_t len)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 // deg of every node of a leaf from LDS counters: a one-piece leaf stores
@@ -900,6 +931,7 @@ size_t degree_work_bytes(uint64_t n, uint32_t d, uint32_t num_cus)
     b += align256(g.len1 * 4) + align256((g.len1 + 1) * 8);
     b += align256(g.len2 * 4) + align256((g.len2 + 1) * 8);
     b += align256(kScanParts * 8) + align256(2 * 8) + align256(((size_t)g.leaves + 1) * 8);
+    b += align256(kHubNodes * 4);
     return b;
 }
 
@@ -922,6 +954,7 @@ void degree_work_layout(void *base, uint64_t n, uint32_t d, uint32_t num_cus, De
     w->partials = (uint64_t *)take(kScanParts * 8);
     w->whole = (uint64_t *)take(2 * 8);
     w->piece_base = (uint64_t *)take(((size_t)g.leaves + 1) * 8);
+    w->hub = (uint32_t *)take(kHubNodes * 4);
     w->num_cus = num_cus;
 }
 
@@ -941,21 +974,24 @@ hipError_t launch_degree_partitioned(const int32_t *G, uint64_t n, uint32_t d, u
     if (g.levels == 0) {
         hipLaunchKernelGGL(deg_whole_kernel, dim3(1), dim3(64), 0, stream, w.whole, g.slots);
     } else if (g.levels == 1) {
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(kHubNodes / kThreads), dim3(kThreads), 0, stream, w.hub, kHubNodes);
         hipLaunchKernelGGL(deg_count_kernel<false>, dim3(W), dim3(kCountThreads), 0, stream, in, g.slots, n, g.shift,
-                           g.buckets, 0u, 0u, w.c1, (uint32_t *)nullptr);
+                           g.buckets, 0u, 0u, kHubNodes, w.c1, (uint32_t *)nullptr, w.hub);
         if ((e = scan_counts<uint32_t>(w.c1, g.len1, w.partials, w.offs1, stream)) != hipSuccess) return e;
-        hipLaunchKernelGGL(deg_scatter1_kernel<true>, dim3(W), dim3(kPartThreads), 0, stream, in, g.slots, n, g.shift,
-                           g.buckets, (const uint64_t *)w.offs1, (int32_t *)nullptr, w.part_b);
+        hipLaunchKernelGGL(deg_scatter1_kernel<true>, dim3(W), dim3(kPartThreads), 0, stream, in, g.slots, n,
+                           kHubNodes, g.shift, g.buckets, (const uint64_t *)w.offs1, (int32_t *)nullptr, w.part_b);
         offs = w.offs1;
         stride = W;
     } else {
-        for (uint32_t b0 = 0; b0 < g.buckets; b0 += kMatBuckets)
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(kHubNodes / kThreads), dim3(kThreads), 0, stream, w.hub, kHubNodes);
+        for (uint32_t b0 = 0; b0 < g.buckets; b0 += kMatBuckets)  // the hub lies in bucket 0
             hipLaunchKernelGGL(deg_count_kernel<true>, dim3(W), dim3(kCountThreads), 0, stream, in, g.slots, n,
-                               g.shift, g.buckets, b0, std::min<uint32_t>(kMatBuckets, g.buckets - b0), w.c1, w.c2);
+                               g.shift, g.buckets, b0, std::min<uint32_t>(kMatBuckets, g.buckets - b0),
+                               b0 == 0 ? kHubNodes : 0u, w.c1, w.c2, w.hub);
         if ((e = scan_counts<uint32_t>(w.c1, g.len1, w.partials, w.offs1, stream)) != hipSuccess) return e;
         if ((e = scan_counts<uint32_t>(w.c2, g.len2, w.partials, w.offs2, stream)) != hipSuccess) return e;
         hipLaunchKernelGGL(deg_scatter1_kernel<false>, dim3(W), dim3(kPartThreads), 0, stream, in, g.slots, n,
-                           g.shift, g.buckets, (const uint64_t *)w.offs1, w.part_a, (uint16_t *)nullptr);
+                           kHubNodes, g.shift, g.buckets, (const uint64_t *)w.offs1, w.part_a, (uint16_t *)nullptr);
         hipLaunchKernelGGL(deg_scatter2_kernel, dim3(W), dim3(kPartThreads), 0, stream, (const int32_t *)w.part_a, n,
                            g.buckets, (const uint64_t *)w.offs1, (const uint64_t *)w.offs2, w.part_b);
         offs = w.offs2;
@@ -966,12 +1002,15 @@ hipError_t launch_degree_partitioned(const int32_t *G, uint64_t n, uint32_t d, u
     hipLaunchKernelGGL(deg_hub_init_kernel, dim3(std::min<uint32_t>(g.leaves, 1024)), dim3(kThreads), 0, stream,
                        (const uint64_t *)w.piece_base, g.leaves, n, d, deg);
     const uint32_t grid = std::max<uint32_t>(1, w.num_cus);
-    if (g.levels == 0)
+    if (g.levels == 0) {
         hipLaunchKernelGGL(deg_leaf_kernel<false>, dim3(grid), dim3(kLeafThreads), 0, stream, (const void *)in, offs,
                            stride, g.leaves, (const uint64_t *)w.piece_base, n, d, deg);
-    else
+    } else {
         hipLaunchKernelGGL(deg_leaf_kernel<true>, dim3(grid), dim3(kLeafThreads), 0, stream, (const void *)w.part_b,
                            offs, stride, g.leaves, (const uint64_t *)w.piece_base, n, d, deg);
+        hipLaunchKernelGGL(deg_hub_add_kernel, dim3(kHubNodes / kThreads), dim3(kThreads), 0, stream,
+                           (const uint32_t *)w.hub, kHubNodes, deg);  // n > 2^15 > kHubNodes here
+    }
     return hipGetLastError();
 }
 
