@@ -17,7 +17,9 @@
 //     buckets x 256 leaves of LDS counters per launch (one launch up to 2^30
 //     nodes, two above). Scanned in (bucket, leaf, workgroup) order, the
 //     matrix places level 2 without a second count pass: level-2 workgroup j
-//     moves exactly the runs that level-1 workgroup j left in the buckets;
+//     moves exactly the runs that level-1 workgroup j left in the buckets.
+//     The same pass counts the 8192 hub nodes (the smallest ids) exactly and
+//     leaves their slots out of the partitions;
 //   * a scatter orders each 8192-slot tile by bin in LDS before storing it,
 //     so each bin's run leaves the workgroup as contiguous stores; the last
 //     level keeps only a key's low 15 bits (16-bit keys, half the bytes);
