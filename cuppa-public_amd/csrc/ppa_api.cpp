@@ -273,6 +273,7 @@ int ppa_ctx_destroy(ppa_ctx *ctx)
 {
     if (!ctx) return PPA_ERR_INVALID;
     DeviceGuard g(ctx->device);
+    if (ctx->pending) (void)hipEventSynchronize(ctx->ev[2]);  // the launch still uses the scratch
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);

@@ -1,6 +1,7 @@
 // ppa_noalloc.cpp -- ppa_generate for callers that cannot allocate device
-// memory (code under stream capture, tight serving loops, harnesses that
-// forbid hipMalloc). It has no counterpart in the reference.
+// memory (tight serving loops, harnesses that forbid hipMalloc, code that must
+// not allocate around stream captures; it waits for its stream, so the call
+// itself is not capturable). It has no counterpart in the reference.
 //
 // The generator's only device memory beyond the caller's graph is one
 // DeviceScratch record per launch: the claim counter, the status word and the

@@ -175,8 +175,10 @@ int ppa_get_stats(ppa_ctx *ctx, ppa_stats *out);
 int ppa_generate(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targets,
                  int64_t *d_row_ptr, void *stream);
 /* No counterpart in the reference. For callers that cannot allocate device
- * memory (code under stream capture, tight serving loops, harnesses that
- * forbid hipMalloc): ppa_generate on the current device without a context,
+ * memory (tight serving loops, harnesses that forbid hipMalloc, code that must
+ * not allocate between or around stream captures; the call waits for its
+ * stream, so it is not itself capturable): ppa_generate on the current device
+ * without a context,
  * with the default launch rules. It allocates nothing: the launch's claim
  * counter, status word and counters (128 bytes) live in a static device
  * buffer of 256 bytes that the library holds on each GPU it runs on until the
