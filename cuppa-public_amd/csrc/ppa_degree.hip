@@ -702,16 +702,22 @@ constexpr uint32_t kWaveShare = 1024;             // slots per wavefront per til
 constexpr uint32_t kSteps = kWaveShare / 64;      // 64-slot steps per wavefront
 
 // Lanes among `act` whose digit equals this lane's; whole wavefront.
+// Per bit: a sign-extended bit field s (0 or all ones), its ballot m, and
+// each half of the mask ANDed with ~(m ^ s) (lanes whose bit equals this
+// lane's): six VALU instructions, where a per-lane select between m and ~m
+// of 64-bit values took ten.
 __device__ __forceinline__ uint64_t match_digit(uint32_t digit, bool act)
 {
-    uint64_t eq = __ballot(act);
+    const uint64_t a = __ballot(act);
+    uint32_t lo = (uint32_t)a, hi = (uint32_t)(a >> 32);
 #pragma unroll
     for (uint32_t b = 0; b < kRadixBits; ++b) {
-        const bool bit = (digit >> b) & 1u;
-        const uint64_t m = __ballot(bit);
-        eq &= bit ? m : ~m;
+        const uint32_t s = (uint32_t)((int32_t)(digit << (31 - b)) >> 31);
+        const uint64_t m = __ballot(s != 0u);
+        lo &= ~((uint32_t)m ^ s);
+        hi &= ~((uint32_t)(m >> 32) ^ s);
     }
-    return eq;
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // The number of valid pairs left by the first pass (offsets[len]) becomes
