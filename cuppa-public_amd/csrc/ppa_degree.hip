@@ -852,8 +852,12 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void ls
 // lane before (or the step before's last lane, or one load for the span's
 // first), and consecutive lanes write consecutive in_ptr entries. (Four
 // consecutive positions per thread with an extra load of the key before them
-// took 14.5 ms at n = 2^30, d = 8.)
+// took 14.5 ms at n = 2^30, d = 8.) A lane whose range (key[i-1], key[i]]
+// holds more than kPtrLane nodes (nodes no slot names: the tail of a graph
+// whose last nodes have no in-edges, or any targets array with few distinct
+// values) leaves it to the whole wavefront, which writes 64 entries a store.
 constexpr uint32_t kPtrSteps = 16;
+constexpr int64_t kPtrLane = 32;
 __global__ __launch_bounds__(256) void csr_ptr_kernel(const int32_t *keys, const uint64_t *total_p, uint64_t n,
                                                       uint64_t cap, int64_t *in_ptr)
 {
@@ -876,8 +880,18 @@ __global__ __launch_bounds__(256) void csr_ptr_kernel(const int32_t *keys, const
             const uint64_t i = s + j * 64 + lane;
             const int32_t up = __shfl_up(k[j], 1, kWave);
             const int32_t prev = lane == 0 ? before : up;
-            if (i <= total)
-                for (int64_t v = (int64_t)prev + 1; v <= (int64_t)k[j]; ++v) in_ptr[v] = (int64_t)i;
+            const int64_t lo = (int64_t)prev + 1, hi = k[j];  // in_ptr[lo..hi] = i
+            const bool wide = i <= total && hi - lo >= kPtrLane;
+            if (i <= total && !wide)
+                for (int64_t v = lo; v <= hi; ++v) in_ptr[v] = (int64_t)i;
+            for (uint64_t m = __ballot(wide); m != 0; m &= m - 1) {
+                const int l = (int)__builtin_ctzll(m);
+                // a wide lane's lo and hi are node ids <= n <= 2^31 - 1
+                const int64_t wlo = __builtin_amdgcn_readlane((int)lo, l);
+                const int64_t whi = __builtin_amdgcn_readlane((int)hi, l);
+                const int64_t wi = (int64_t)(s + j * 64) + l;
+                for (int64_t v = wlo + lane; v <= whi; v += 64) in_ptr[v] = wi;
+            }
             before = __shfl(k[j], 63, kWave);
         }
     }

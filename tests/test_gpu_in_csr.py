@@ -57,3 +57,25 @@ def test_undirected_rejects_incomplete(ctx):
     G[5] = -1
     with pytest.raises(cuppa.PPAError):
         ctx.undirected_csr(cuppa.Graph(n, d, torch.from_numpy(G).cuda(), None))
+
+
+@pytest.mark.parametrize("case", ["few_values", "all_zero", "none_valid"])
+def test_in_csr_wide_gaps(ctx, oracle_mod, case):
+    """Targets with few distinct values leave long runs of nodes no slot
+    names; the whole wavefront fills those runs of in_ptr."""
+    import torch
+    import cuppa
+    n, d = 3_000_000, 2
+    rng = np.random.default_rng(21)
+    if case == "few_values":
+        G = rng.choice(np.array([0, 17, 50, 1000, 999_999, n - 1], np.int32), n * d)
+    elif case == "all_zero":
+        G = np.zeros(n * d, np.int32)
+    else:
+        G = np.full(n * d, -1, np.int32)
+    g = cuppa.Graph(n, d, torch.from_numpy(G).cuda(), None)
+    want_ptr, want_src = oracle_mod.in_csr(n, d, G)
+    ptr, src = ctx.in_csr(g)
+    assert np.array_equal(ptr.cpu().numpy(), want_ptr)
+    total = int(want_ptr[-1])
+    assert np.array_equal(src.cpu().numpy()[:total], want_src[:total])
