@@ -496,12 +496,34 @@ def run_wide(args):
                       "warmup": args.warmup, "data": "synthetic (keyed Philox draws)", "graphs": out}))
 
 
-def degree_and_csr(ctx, n, d, edges, targets, row_ptr) -> dict:
+def cpu_degree_csr_baselines(targets, d: int, nodes: int):
+    """The oracle's degree count (the reference's host loop of
+    cuPPA.cu:103-165, restated) and its in-edge counting sort, one core each,
+    over the first `nodes` nodes of the generated graph: their slots name only
+    nodes below them, so they form a graph of their own."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    slots = nodes * d
+    G = targets[:slots].cpu().numpy()
+    sample = f"oracle restatement, first {nodes} nodes ({slots} slots) of the generated graph"
+    out = {}
+    for key, fn in (("degree", lambda: oracle.degrees(nodes, d, G, 1 << 20)),
+                    ("in_csr", lambda: oracle.in_csr(nodes, d, G))):
+        t0 = time.perf_counter()
+        fn()
+        secs = time.perf_counter() - t0
+        out[key] = {"value": slots / secs, "unit": "slots/s", "cores": 1, "kind": "port", "sample": sample,
+                    "seconds": secs, "host": host_info()}
+    return out
+
+
+def degree_and_csr(ctx, n, d, edges, targets, row_ptr, cpu_nodes: int = 0) -> dict:
     """The degree pass and the in-edge CSR of the generated graph: the first
     call of each also allocates the context's work memory; the second is
     timed. Algorithmic bytes of the degree pass: 4 B/slot read + 8 B/slot
     degree update + 4 B/node histogram read; of the in-edge CSR: 4 B/slot
-    read + 4 B/slot in_src write + 8 B/node in_ptr write."""
+    read + 4 B/slot in_src write + 8 B/node in_ptr write. cpu_nodes > 0 adds
+    the oracle's one-core rates over that many nodes as cpu_baseline."""
     import torch
     import cuppa
     out = {}
@@ -516,6 +538,7 @@ def degree_and_csr(ctx, n, d, edges, targets, row_ptr) -> dict:
     del hist
     alg = 12.0 * edges + 4.0 * n
     out["degree"] = {"ms": times[1], "first_call_ms": times[0], "max_degree": mx,
+                     "slots_per_s": edges / (times[1] * 1e-3),
                      "algorithmic_gbs": alg / (times[1] * 1e-3) / 1e9,
                      "roofline_frac": alg / (times[1] * 1e-3) / 1e9 / HBM_PEAK_GBS}
     times = []
@@ -532,6 +555,9 @@ def degree_and_csr(ctx, n, d, edges, targets, row_ptr) -> dict:
                      "roofline_frac": alg / (times[1] * 1e-3) / 1e9 / HBM_PEAK_GBS}
     ctx.trim()  # give the work memory back before the next measurements
     torch.cuda.empty_cache()
+    if cpu_nodes:
+        for key, base in cpu_degree_csr_baselines(targets, d, min(cpu_nodes, n)).items():
+            out[key]["cpu_baseline"] = base
     return out
 
 
@@ -664,7 +690,8 @@ def main():
 
     extra = {}
     if not args.no_degree:
-        extra.update(degree_and_csr(ctx, n, d, edges, targets, row_ptr))
+        extra.update(degree_and_csr(ctx, n, d, edges, targets, row_ptr,
+                                    cpu_nodes=args.cpu_prefix))
     if args.ba:
         # same n, d and buffers; the first call is the warm-up
         ba_desc = cuppa.make_desc(n, d, seed=seed, model=cuppa.MODEL_BA)
