@@ -443,9 +443,10 @@ int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t
     if (ensure_work(ctx, degree_work_bytes(desc->n, desc->d, (uint32_t)ctx->num_cus), stream) == PPA_OK) {
         DegreeWork w;
         degree_work_layout(ctx->work, desc->n, desc->d, (uint32_t)ctx->num_cus, &w);
+        w.status = &ctx->scratch->status;
         PPA_HIP(launch_degree_partitioned(d_targets, desc->n, desc->d, deg, w, stream));
     } else {
-        PPA_HIP(launch_degree_atomic(d_targets, desc->n, desc->d, deg, stream));
+        PPA_HIP(launch_degree_atomic(d_targets, desc->n, desc->d, deg, &ctx->scratch->status, stream));
     }
     PPA_HIP(launch_seed_degree(deg, desc->d, desc->model, stream));
     PPA_HIP(launch_degree_histogram(deg, desc->n, d_hist, hist_len, ctx->scratch, stream));
@@ -453,7 +454,9 @@ int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t
         PPA_HIP(hipMemcpyAsync(d_max_degree, &ctx->scratch->max_degree, sizeof(uint64_t),
                                hipMemcpyDeviceToDevice, stream));
     PPA_HIP(hipStreamSynchronize(stream));
-    return PPA_OK;
+    int32_t status = 0;  // PPA_ERR_RANGE from a degree that passed 2^32 - 1
+    PPA_HIP(hipMemcpy(&status, &ctx->scratch->status, sizeof(status), hipMemcpyDeviceToHost));
+    return status;
 }
 
 int ppa_histogram_of_degrees(ppa_ctx *ctx, uint64_t n, const uint32_t *d_degree, uint64_t *d_hist,

@@ -66,6 +66,7 @@ const char *ppa_strerror(int code)
     case PPA_ERR_CANCELLED: return "request cancelled";
     case PPA_ERR_AGAIN: return "not ready, try again";
     case PPA_ERR_INTERNAL: return "internal error";
+    case PPA_ERR_RANGE: return "a result exceeds its output type (degree above 2^32 - 1)";
     default: return "unknown error";
     }
 }

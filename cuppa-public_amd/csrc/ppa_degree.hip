@@ -351,7 +351,7 @@ template <bool MAT>
 __global__ __launch_bounds__(kCountThreads) void deg_count_kernel(const int32_t *in, uint64_t slots, uint64_t n,
                                                                   uint32_t shift, uint32_t buckets, uint32_t b_lo,
                                                                   uint32_t nb, uint32_t hub_n, uint32_t *c1,
-                                                                  uint32_t *cnt2, uint32_t *hub)
+                                                                  uint32_t *cnt2, uint32_t *hub, int32_t *status)
 {
     constexpr uint32_t kCells = MAT ? kMatBuckets * kSubBins : kMaxBuckets;
     constexpr int kT = kCountThreads * kPerThread;
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(kCountThreads) void deg_count_kernel(const int32_t 
     if ((threadIdx.x & 63u) == 0 && hot_hub) atomicAdd(&hubc[0], hot_hub);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < hub_n; i += kCountThreads)
-        if (hubc[i]) atomicAdd(&hub[i], hubc[i]);
+        if (hubc[i]) check_u32_add(atomicAdd(&hub[i], hubc[i]), hubc[i], status);
     if (MAT) {
         for (uint32_t i = threadIdx.x; i < ncell; i += kCountThreads)
             cnt2[((uint64_t)b_lo * kSubBins + i) * W + w] = cell[i];
@@ -596,10 +596,12 @@ __global__ void deg_hub_init_kernel(const uint64_t *piece_base, uint32_t leaves,
 
 // The hub nodes' counts from the count pass, added after the leaf counts
 // (which never saw their slots).
-__global__ void deg_hub_add_kernel(const uint32_t *hub, uint32_t hub_n, uint32_t *deg)
+__global__ void deg_hub_add_kernel(const uint32_t *hub, uint32_t hub_n, uint32_t *deg, int32_t *status)
 {
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < hub_n; v += gridDim.x * blockDim.x)
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < hub_n; v += gridDim.x * blockDim.x) {
+        check_u32_add(deg[v], hub[v], status);
         deg[v] += hub[v];
+    }
 }
 
 __global__ void zero_u32_kernel(uint32_t *p, uint32_t len)
@@ -616,7 +618,7 @@ template <bool K16>
 __global__ __launch_bounds__(kLeafThreads) void deg_leaf_kernel(const void *keys, const uint64_t *offs,
                                                                uint32_t stride, uint32_t leaves,
                                                                const uint64_t *piece_base, uint64_t n, uint32_t d,
-                                                               uint32_t *deg)
+                                                               uint32_t *deg, int32_t *status)
 {
     __shared__ uint32_t cnt[kLeafNodes];
     const uint64_t pieces = piece_base[leaves];
@@ -677,8 +679,8 @@ __global__ __launch_bounds__(kLeafThreads) void deg_leaf_kernel(const void *keys
         for (uint32_t i = threadIdx.x; i < kLeafNodes; i += kLeafThreads) {
             const uint64_t v = base + i;
             if (v < n) {
-                if (np == 1) deg[v] = d + cnt[i];
-                else if (cnt[i]) atomicAdd(&deg[v], cnt[i]);
+                if (np == 1) deg[v] = d + cnt[i];  // at most kPiece slots: no overflow
+                else if (cnt[i]) check_u32_add(atomicAdd(&deg[v], cnt[i]), cnt[i], status);
             }
         }
         __syncthreads();
@@ -1022,7 +1024,7 @@ hipError_t launch_degree_partitioned(const int32_t *G, uint64_t n, uint32_t d, u
     } else if (g.levels == 1) {
         hipLaunchKernelGGL(zero_u32_kernel, dim3(kHubNodes / kThreads), dim3(kThreads), 0, stream, w.hub, kHubNodes);
         hipLaunchKernelGGL(deg_count_kernel<false>, dim3(W), dim3(kCountThreads), 0, stream, in, g.slots, n, g.shift,
-                           g.buckets, 0u, 0u, kHubNodes, w.c1, (uint32_t *)nullptr, w.hub);
+                           g.buckets, 0u, 0u, kHubNodes, w.c1, (uint32_t *)nullptr, w.hub, w.status);
         if ((e = scan_counts<uint32_t>(w.c1, g.len1, w.partials, w.offs1, stream)) != hipSuccess) return e;
         hipLaunchKernelGGL(deg_scatter1_kernel<true>, dim3(W), dim3(kPartThreads), 0, stream, in, g.slots, n,
                            kHubNodes, g.shift, g.buckets, (const uint64_t *)w.offs1, (int32_t *)nullptr, w.part_b);
@@ -1033,7 +1035,7 @@ hipError_t launch_degree_partitioned(const int32_t *G, uint64_t n, uint32_t d, u
         for (uint32_t b0 = 0; b0 < g.buckets; b0 += kMatBuckets)  // the hub lies in bucket 0
             hipLaunchKernelGGL(deg_count_kernel<true>, dim3(W), dim3(kCountThreads), 0, stream, in, g.slots, n,
                                g.shift, g.buckets, b0, std::min<uint32_t>(kMatBuckets, g.buckets - b0),
-                               b0 == 0 ? kHubNodes : 0u, w.c1, w.c2, w.hub);
+                               b0 == 0 ? kHubNodes : 0u, w.c1, w.c2, w.hub, w.status);
         if ((e = scan_counts<uint32_t>(w.c1, g.len1, w.partials, w.offs1, stream)) != hipSuccess) return e;
         if ((e = scan_counts<uint32_t>(w.c2, g.len2, w.partials, w.offs2, stream)) != hipSuccess) return e;
         hipLaunchKernelGGL(deg_scatter1_kernel<false>, dim3(W), dim3(kPartThreads), 0, stream, in, g.slots, n,
@@ -1050,12 +1052,12 @@ hipError_t launch_degree_partitioned(const int32_t *G, uint64_t n, uint32_t d, u
     const uint32_t grid = std::max<uint32_t>(1, w.num_cus);
     if (g.levels == 0) {
         hipLaunchKernelGGL(deg_leaf_kernel<false>, dim3(grid), dim3(kLeafThreads), 0, stream, (const void *)in, offs,
-                           stride, g.leaves, (const uint64_t *)w.piece_base, n, d, deg);
+                           stride, g.leaves, (const uint64_t *)w.piece_base, n, d, deg, w.status);
     } else {
         hipLaunchKernelGGL(deg_leaf_kernel<true>, dim3(grid), dim3(kLeafThreads), 0, stream, (const void *)w.part_b,
-                           offs, stride, g.leaves, (const uint64_t *)w.piece_base, n, d, deg);
+                           offs, stride, g.leaves, (const uint64_t *)w.piece_base, n, d, deg, w.status);
         hipLaunchKernelGGL(deg_hub_add_kernel, dim3(kHubNodes / kThreads), dim3(kThreads), 0, stream,
-                           (const uint32_t *)w.hub, kHubNodes, deg);  // n > 2^15 > kHubNodes here
+                           (const uint32_t *)w.hub, kHubNodes, deg, w.status);  // n > 2^15 > kHubNodes here
     }
     return hipGetLastError();
 }

@@ -20,6 +20,7 @@ struct DegreeWork {
     uint64_t *whole = nullptr;      // {0, slots}: the one leaf of a graph of at most 2^15 nodes
     uint64_t *piece_base = nullptr; // exclusive scan of pieces per leaf
     uint32_t *hub = nullptr;        // in-slot counts of the hub nodes, from the count pass
+    int32_t *status = nullptr;      // set to -11 (PPA_ERR_RANGE) if a degree passes 2^32 - 1
     uint64_t *counts = nullptr;     // in-edge CSR: per (digit, workgroup) counts
     uint64_t *offsets = nullptr;    // in-edge CSR: their exclusive scan
     uint32_t num_cus = 0;           // compute units the launch shapes are sized for

@@ -11,6 +11,14 @@ constexpr uint32_t kMaxRetry = 1u << 24;
 constexpr int kErrRetry = -2;
 constexpr int kErrTimeout = -5;
 constexpr int kErrInternal = -10;
+constexpr int kErrRange = -11;
+
+// A 32-bit count that would pass 2^32 - 1 when `add` is added to `old` (the
+// value before an atomic add) is reported through *status, first error wins.
+__device__ __forceinline__ void check_u32_add(uint32_t old, uint32_t add, int32_t *status)
+{
+    if (old > 0xFFFFFFFFu - add) atomicCAS(status, 0, kErrRange);
+}
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 

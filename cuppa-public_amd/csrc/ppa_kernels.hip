@@ -1514,12 +1514,12 @@ __global__ void degree_init_kernel(uint32_t *deg, uint64_t n, uint32_t d)
 // cannot get its buffers. Out-of-range values (an incomplete graph) are
 // skipped, never dereferenced.
 __global__ void degree_count_kernel(const int32_t *G, uint64_t first_slot, uint64_t slots, uint64_t n,
-                                    uint32_t *deg)
+                                    uint32_t *deg, int32_t *status)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t s = first_slot + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += stride) {
         const int32_t v = G[s];
-        if (v >= 0 && (uint64_t)v < n) atomicAdd(&deg[v], 1u);
+        if (v >= 0 && (uint64_t)v < n) check_u32_add(atomicAdd(&deg[v], 1u), 1u, status);
     }
 }
 
@@ -1722,13 +1722,14 @@ hipError_t generate_occupancy(uint32_t d, uint32_t threads, int *out)
     }
 }
 
-hipError_t launch_degree_atomic(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, hipStream_t stream)
+hipError_t launch_degree_atomic(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, int32_t *status,
+                                hipStream_t stream)
 {
     hipLaunchKernelGGL(degree_init_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, stream, deg, n, d);
     const uint64_t first = (uint64_t)(d + 1) * d, slots = n * d;
     if (slots > first)
         hipLaunchKernelGGL(degree_count_kernel, dim3(grid_for(slots - first, 256, 16384)), dim3(256), 0,
-                           stream, G, first, slots, n, deg);
+                           stream, G, first, slots, n, deg, status);
     return hipGetLastError();
 }
 

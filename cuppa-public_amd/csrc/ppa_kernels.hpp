@@ -174,7 +174,8 @@ hipError_t launch_prefix(uint32_t d, const PrefixArgs &a, hipStream_t stream);
 hipError_t launch_prefix_batch(const PrefixArgs &a, uint32_t count, hipStream_t stream);
 hipError_t launch_generate_batch(const GenArgs &a, uint32_t blocks, uint32_t block_threads, hipStream_t stream);
 hipError_t generate_occupancy(uint32_t d, uint32_t block_threads, int *blocks_per_cu);
-hipError_t launch_degree_atomic(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, hipStream_t stream);
+hipError_t launch_degree_atomic(const int32_t *G, uint64_t n, uint32_t d, uint32_t *deg, int32_t *status,
+                                hipStream_t stream);
 // Adjusts the seed nodes' degrees of a non-copy model (one workgroup).
 hipError_t launch_seed_degree(uint32_t *deg, uint32_t d, uint32_t model, hipStream_t stream);
 hipError_t launch_degree_histogram(const uint32_t *deg, uint64_t n, uint64_t *hist, uint64_t hist_len,

@@ -28,7 +28,7 @@ PPA_MAX_RETRY = 1 << 24
 ERRORS = {
     -1: "PPA_ERR_INVALID", -2: "PPA_ERR_RETRY", -3: "PPA_ERR_NOMEM", -4: "PPA_ERR_DEVICE",
     -5: "PPA_ERR_TIMEOUT", -6: "PPA_ERR_BUDGET", -7: "PPA_ERR_NOT_FOUND", -8: "PPA_ERR_CANCELLED",
-    -9: "PPA_ERR_AGAIN", -10: "PPA_ERR_INTERNAL",
+    -9: "PPA_ERR_AGAIN", -10: "PPA_ERR_INTERNAL", -11: "PPA_ERR_RANGE",
 }
 
 

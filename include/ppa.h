@@ -49,7 +49,8 @@ enum {
     PPA_ERR_NOT_FOUND = -7, /* unknown request id */
     PPA_ERR_CANCELLED = -8, /* request was cancelled */
     PPA_ERR_AGAIN = -9,     /* not ready yet / queue full, try again */
-    PPA_ERR_INTERNAL = -10
+    PPA_ERR_INTERNAL = -10,
+    PPA_ERR_RANGE = -11     /* a result exceeds its output type (a degree above 2^32 - 1) */
 };
 
 #define PPA_MAX_RETRY (1u << 24)
@@ -223,7 +224,10 @@ int ppa_generate_range(ppa_ctx *ctx, const ppa_graph_desc *desc, int32_t *d_targ
  * leaves out the seed self-loops. d_degree (nullable,
  * uint32[n]) receives deg; d_hist (nullable, uint64[hist_len]) receives
  * hist[x] = #{v : deg(v) == x} for x < hist_len; d_max_degree (nullable,
- * one uint64) receives max deg. Waits for the stream. */
+ * one uint64) receives max deg. Waits for the stream. PPA_ERR_RANGE if a
+ * degree exceeds 2^32 - 1 (the reference counts in int64): only a graph of
+ * more than 2^32 slots can, when over 2^32 - 1 - d of them name one node;
+ * the outputs then hold that degree modulo 2^32. */
 int ppa_degree_histogram(ppa_ctx *ctx, const ppa_graph_desc *desc, const int32_t *d_targets,
                          uint32_t *d_degree, uint64_t *d_hist, uint64_t hist_len,
                          uint64_t *d_max_degree, void *stream);

@@ -83,6 +83,36 @@ def test_degrees_above_2p30_nodes(ctx):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("node", [0, 10_000, 5_000_000])
+@pytest.mark.parametrize("extra", [7, 8])
+def test_degree_past_32_bits_is_reported(ctx, node, extra):
+    """Every slot names one node, so deg = d + (n - d - 1) * d = 8 * (n - 8)
+    at d = 8: 2^32 - 8 at n = 2^29 + 7, which the 32-bit degree holds, and
+    2^32 at n = 2^29 + 8, which it cannot: that call returns PPA_ERR_RANGE
+    instead of the count modulo 2^32. Node 0 is one of the hub nodes the
+    count pass counts itself; nodes 10,000 and 5,000,000 are counted in leaves
+    split into pieces (added to the degree with atomics)."""
+    import torch
+    import cuppa
+    d = 8
+    n = (1 << 29) + extra
+    G = torch.full((n * d,), node, dtype=torch.int32, device="cuda")
+    g = cuppa.Graph(n, d, G, None)
+    if extra == 8:
+        with pytest.raises(cuppa.PPAError) as e:
+            ctx.degree_histogram(g, hist_len=16)
+        assert e.value.code == -11
+    else:
+        hist, mx, deg = ctx.degree_histogram(g, hist_len=16, want_degree=True)
+        assert mx == (1 << 32) - 8
+        assert int(deg[node].item()) & 0xFFFFFFFF == (1 << 32) - 8
+        assert int(hist[d].item()) == n - 1  # every other node keeps its d out-slots
+        del deg, hist
+    del G, g
+    ctx.trim()
+    torch.cuda.empty_cache()
+
+
 def test_histogram_of_degrees(ctx):
     """ppa_histogram_of_degrees over degrees left by ppa_degree_histogram:
     same histogram and maximum, truncated below hist_len, maximum alone with
