@@ -149,6 +149,7 @@ int ppa_cpu_degree_histogram(const ppa_graph_desc *desc, const int32_t *targets,
     for (uint64_t s = (uint64_t)(d + 1) * d; s < n * d; ++s) {
         const int32_t v = targets[s];
         if (v < 0 || (uint64_t)v >= n) return PPA_ERR_INVALID;
+        if (degree[v] == UINT32_MAX) return PPA_ERR_RANGE;  // the degree would pass 2^32 - 1
         degree[v]++;
     }
     if (hist_len) std::memset(hist, 0, hist_len * sizeof(uint64_t));

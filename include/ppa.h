@@ -387,8 +387,9 @@ int ppa_server_replay(ppa_server *srv, const ppa_request *reqs, const double *ar
  * retries (nullable) receives the number of rejected attempts. */
 int ppa_cpu_generate(const ppa_graph_desc *desc, int32_t *targets, int64_t *row_ptr, uint64_t *retries);
 /* Host-memory degree pass of PAHelper::outputSCMDegreeDistribution
- * (PAHelper.hpp:15-67) with deg(v) as in ppa_degree_histogram. degree is
- * uint32[n] scratch supplied by the caller. */
+ * (PAHelper.hpp:15-67) with deg(v) as in ppa_degree_histogram, and
+ * PPA_ERR_RANGE as there. degree is uint32[n] scratch supplied by the
+ * caller. */
 int ppa_cpu_degree_histogram(const ppa_graph_desc *desc, const int32_t *targets, uint32_t *degree,
                              uint64_t *hist, uint64_t hist_len, uint64_t *max_degree);
 
