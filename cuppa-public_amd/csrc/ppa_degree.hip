@@ -60,6 +60,7 @@ constexpr int kPartPer = 16;                  // slots per scatter thread per ti
 constexpr int kPartTile = kPartThreads * kPartPer;
 constexpr uint32_t kPartPerCU = 4;            // partition workgroups per CU (8 measured 2 % slower)
 constexpr int kLeafThreads = 1024;
+constexpr int kLeafUnroll = 4;                // 16-byte key loads in flight per leaf-count thread (8, 16: no faster)
 constexpr uint64_t kPiece = 1ull << 20;       // slots of one leaf-count work item
 constexpr int kPlanThreads = 1024;
 
@@ -648,15 +649,15 @@ __global__ __launch_bounds__(kLeafThreads) void deg_leaf_kernel(const void *keys
             const uint64_t a1 = std::max<uint64_t>(a0, end & ~7ull);           // 16-byte body [a0, a1)
             for (uint64_t i = first + threadIdx.x; i < a0; i += kLeafThreads) count(k16[i]);
             const uint64_t g1 = a1 / 8;
-            for (uint64_t g = a0 / 8 + threadIdx.x; g < g1; g += 4 * kLeafThreads) {
-                uint4 x[4];
+            for (uint64_t g = a0 / 8 + threadIdx.x; g < g1; g += kLeafUnroll * kLeafThreads) {
+                uint4 x[kLeafUnroll];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kLeafUnroll; ++u) {
                     const uint64_t gg = g + (uint64_t)u * kLeafThreads;
                     x[u] = gg < g1 ? k4[gg] : make_uint4(~0u, ~0u, ~0u, ~0u);  // 0xffff: no key
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kLeafUnroll; ++u) {
                     const uint32_t wd[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
 #pragma unroll
                     for (int h = 0; h < 4; ++h) {
