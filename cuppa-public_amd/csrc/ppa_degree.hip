@@ -20,7 +20,7 @@
 //     moves exactly the runs that level-1 workgroup j left in the buckets.
 //     The same pass counts the 8192 hub nodes (the smallest ids) exactly and
 //     leaves their slots out of the partitions;
-//   * a scatter orders each 8192-slot tile by bin in LDS before storing it,
+//   * a scatter orders each 12288-slot tile by bin in LDS before storing it,
 //     so each bin's run leaves the workgroup as contiguous stores; the last
 //     level keeps only a key's low 15 bits (16-bit keys, half the bytes);
 //   * a leaf of at most kPiece slots is counted by one workgroup, which
@@ -55,8 +55,8 @@ constexpr uint32_t kMatBuckets = 128;         // buckets one matrix count covers
 constexpr uint32_t kHubNodes = 8192;          // nodes the count pass counts itself (32 KB of LDS)
 constexpr int kCountThreads = 1024;           // count workgroups: one per CU by LDS, 16 waves
 constexpr int kCountPer = 32;                 // slots per count thread and tile (16: 0.35 ms slower at 2^33 slots)
-constexpr int kPartThreads = 512;             // scatter workgroups: 8192-slot tiles
-constexpr int kPartPer = 16;                  // slots per scatter thread per tile
+constexpr int kPartThreads = 512;             // scatter workgroups: 12288-slot tiles
+constexpr int kPartPer = 24;                  // slots per scatter thread per tile (16, 28, 32: see below)
 constexpr int kPartTile = kPartThreads * kPartPer;
 constexpr uint32_t kPartPerCU = 4;            // partition workgroups per CU (8 measured 2 % slower)
 constexpr int kLeafThreads = 1024;
@@ -431,9 +431,13 @@ __global__ __launch_bounds__(kCountThreads) void deg_count_kernel(const int32_t 
 }
 
 // Moves the valid keys of in[lo, hi) to their bins' runs of the output, one
-// 8192-slot tile at a time: the tile is grouped by bin ((v >> shift) & mask,
+// 12288-slot tile at a time: the tile is grouped by bin ((v >> shift) & mask,
 // at most 256 bins) in LDS first, so each bin's run leaves the workgroup as
-// contiguous stores (at 128-256 bins a run averages 32-64 slots). cur[] holds
+// contiguous stores (at 128-256 bins a run averages 48-96 slots). At
+// n = 2^30, d = 8 the two scatters took 18.3 + 17.3 ms with 16 slots per
+// thread (8192-slot tiles), 17.3 + 15.8 with 24, 17.4 + 15.4 with 28 (125
+// VGPRs, at the edge of four wavefronts per SIMD) and 26.9 + 22.5 with 32
+// (143 VGPRs). cur[] holds
 // each bin's next output position and thist[] must be zero on entry; OUT16
 // stores a key's low 15 bits (the last level), otherwise the whole key. Bin
 // 0 is ranked with one LDS atomic per wavefront. Ends with a barrier.
