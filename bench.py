@@ -561,6 +561,31 @@ def degree_and_csr(ctx, n, d, edges, targets, row_ptr, cpu_nodes: int = 0) -> di
     return out
 
 
+def design_floors(extra: dict, n: int, d: int, edges: int, copy_gbs: float) -> None:
+    """The bytes each pass's own design moves through HBM, and the time a
+    device copy's rate (copy_gbs, measured in the same run) needs for them:
+    how far the degree pass and the in-edge CSR are from the bound of the way
+    they work, beside their algorithmic roofline. Degree pass with two
+    partition levels (n > 2^23): count 4 + level-1 scatter 4 + 4 + level-2
+    scatter 4 + 2 + leaf count 2 B/slot (the hub nodes' slots, which skip
+    both scatters, not taken off), 4 B/node degree write + 4 B/node histogram
+    read. In-edge CSR with P = ceil(bits(n - 1) / 8) passes: P counts of 4
+    B/slot, a first scatter of 4 + 8 B/slot, P - 1 scatters of 8 + 8 B/slot,
+    the in_ptr read of 4 B/slot, 8 B/node in_ptr write."""
+    bits = max(1, (n - 1).bit_length())
+    passes = (bits + 7) // 8
+    plans = {"degree": 20.0 * edges + 8.0 * n if n > (1 << 23) else None,
+             "in_csr": (4.0 * passes + 12.0 + 16.0 * (passes - 1) + 4.0) * edges + 8.0 * n}
+    for key, nbytes in plans.items():
+        obj = extra.get(key)
+        if obj is None or nbytes is None:
+            continue
+        floor_ms = nbytes / (copy_gbs * 1e9) * 1e3
+        obj["design_bytes"] = nbytes
+        obj["design_floor_ms"] = floor_ms
+        obj["frac_of_design_floor"] = floor_ms / obj["ms"]
+
+
 def profile_kernel_ms(profile_dir: str, workload: str):
     """The generation kernel's average duration in the committed PMC summary
     (the run its traffic figures come from), or None."""
@@ -723,6 +748,7 @@ def main():
     del targets, row_ptr
     torch.cuda.empty_cache()
     copy_gbs = copy_bandwidth_gbs(torch)
+    design_floors(extra, n, d, edges, copy_gbs)
     requests = read_requests(profiles, args.workload)
     request_roof = None
     if requests and not args.no_gather_ceiling:
