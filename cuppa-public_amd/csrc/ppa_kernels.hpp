@@ -232,7 +232,12 @@ inline WideGeometry wide_geometry(uint32_t d)
     }
     g.streams = d <= 96 ? 8u : 4u;
     g.claim = d <= 40 ? 16u : (d <= 64 ? 8u : (d <= 128 ? 4u : 2u));
-    g.srows = 4u;
+    // LDS bounds the wavefronts per CU (one per 64-thread workgroup), and the
+    // stream rows are much of it: 4 rows a stream held 8 workgroups per CU at
+    // d = 33 (20 KB each) and 6 at d = 96 (24 KB); 1 row at d <= 40 and 2
+    // above fit 11 and 8: d = 33 57.7 -> 51.1 ms (53.9 with 2 rows), d = 96
+    // 70.6 -> 61.0, d = 48, 64 and 128 within 1 % (n * d = 2^30 .. 2^31).
+    g.srows = d <= 40 ? 1u : 2u;
     g.prows = d > 64 ? 12u : 0u;
     const uint32_t by_lds = 8192u >> g.hbits;  // hbits <= 9 here
     const uint32_t want = g.claim + 1u < 8u ? g.claim + 1u : 8u;
