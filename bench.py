@@ -459,8 +459,8 @@ def run_latency(args):
 
 # Out-degrees above 32 (the wavefront-stream generator), at 2^30 .. 2^31
 # slots, beside d = 32 on the lane-per-node generator.
-WIDE_GRAPHS = [(1 << 25, 32, 0.2), (1 << 25, 33, 0.2), (1 << 24, 64, 0.2), (1 << 23, 128, 0.2),
-               (1 << 22, 256, 0.2), (1 << 20, 1024, 0.2), (1 << 17, 8192, 0.5)]
+WIDE_GRAPHS = [(1 << 25, 32, 0.2), (1 << 25, 33, 0.2), (1 << 24, 64, 0.2), (1 << 23, 96, 0.2),
+               (1 << 23, 128, 0.2), (1 << 22, 256, 0.2), (1 << 20, 1024, 0.2), (1 << 17, 8192, 0.5)]
 
 
 def run_wide(args):
