@@ -60,7 +60,8 @@ constexpr int kPartPer = 24;                  // slots per scatter thread per ti
 constexpr int kPartTile = kPartThreads * kPartPer;
 constexpr uint32_t kPartPerCU = 4;            // partition workgroups per CU (8 measured 2 % slower)
 constexpr int kLeafThreads = 1024;
-constexpr int kLeafUnroll = 4;                // 16-byte key loads in flight per leaf-count thread (8, 16: no faster)
+constexpr int kLeafUnroll = 4;                // 16-byte key loads in flight per leaf-count thread (8, 16: no faster;
+                                              // a ballot per key to add equal keys at once: 4.3 -> 7.0 ms)
 constexpr uint64_t kPiece = 1ull << 20;       // slots of one leaf-count work item
 constexpr int kPlanThreads = 1024;
 
