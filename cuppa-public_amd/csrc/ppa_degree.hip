@@ -866,7 +866,7 @@ __global__ __launch_bounds__(TH) __attribute__((amdgpu_waves_per_eu(4))) void ls
 // holds more than kPtrLane nodes (nodes no slot names: the tail of a graph
 // whose last nodes have no in-edges, or any targets array with few distinct
 // values) leaves it to the whole wavefront, which writes 64 entries a store.
-constexpr uint32_t kPtrSteps = 16;
+constexpr uint32_t kPtrSteps = 16;  // 9.2 ms at n = 2^30, d = 8 (8 steps: 9.4, 32: 10.1)
 constexpr int64_t kPtrLane = 32;
 __global__ __launch_bounds__(256) void csr_ptr_kernel(const int32_t *keys, const uint64_t *total_p, uint64_t n,
                                                       uint64_t cap, int64_t *in_ptr)
