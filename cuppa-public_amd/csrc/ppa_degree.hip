@@ -58,7 +58,8 @@ constexpr int kCountPer = 32;                 // slots per count thread and tile
 constexpr int kPartThreads = 512;             // scatter workgroups: 12288-slot tiles
 constexpr int kPartPer = 24;                  // slots per scatter thread per tile (16, 28, 32: see below)
 constexpr int kPartTile = kPartThreads * kPartPer;
-constexpr uint32_t kPartPerCU = 4;            // partition workgroups per CU (8 measured 2 % slower)
+constexpr uint32_t kPartPerCU = 4;            // partition workgroups per CU (8 measured 2 % slower;
+                                              // 2: 46.0-46.4 ms against 46.2-48.5, inside the spread)
 constexpr int kLeafThreads = 1024;
 constexpr int kLeafUnroll = 4;                // 16-byte key loads in flight per leaf-count thread (8, 16: no faster;
                                               // a ballot per key to add equal keys at once: 4.3 -> 7.0 ms)
